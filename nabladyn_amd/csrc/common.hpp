@@ -1,0 +1,154 @@
+// Shared definitions of the gfx950 kernels: index type, status bits, the rebuild gate, wave64
+// helpers, a block-wide scan, and the C-ABI error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/nabladyn_amd.h"
+
+namespace ndn {
+
+using index_t = std::int32_t;
+
+constexpr int kWave = 64;        // CDNA wavefront
+constexpr int kBlock = 256;      // 4 waves
+constexpr int kStatusInvalidCell = NDN_STATUS_INVALID_CELL;
+constexpr int kStatusNonFinite = NDN_STATUS_NON_FINITE;
+constexpr int kStatusOverflow = NDN_STATUS_OVERFLOW;
+constexpr int kStatusFullOverflow = NDN_STATUS_FULL_OVERFLOW;
+
+inline unsigned int blocks_for(std::int64_t items, int threads = kBlock)
+{
+    return static_cast<unsigned int>((items + threads - 1) / threads);
+}
+
+// Whether a rebuild kernel works: a first-frame build passes no flag and always does; an update
+// works only when the reuse check asked for a rebuild and no status bit is set.
+__device__ __forceinline__ bool rebuilding(const index_t* rebuild, const index_t* status)
+{
+    return rebuild == nullptr || (rebuild[0] != 0 && status[0] == 0);
+}
+
+__device__ __forceinline__ int lane_id()
+{
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int bits_below(std::uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+}
+
+__device__ __forceinline__ std::uint64_t ballot(bool predicate)
+{
+    return __ballot(predicate);
+}
+
+template <typename V>
+__device__ __forceinline__ V wave_inclusive_sum(V value)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int offset = 1; offset < kWave; offset <<= 1)
+    {
+        const V other = __shfl_up(value, offset, kWave);
+        if (lane >= offset) value += other;
+    }
+    return value;
+}
+
+template <typename V>
+__device__ __forceinline__ V wave_sum(V value)
+{
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    {
+        value += __shfl_xor(value, offset, kWave);
+    }
+    return value;
+}
+
+template <typename V>
+__device__ __forceinline__ V wave_max(V value)
+{
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    {
+        const V other = __shfl_xor(value, offset, kWave);
+        value = other > value ? other : value;
+    }
+    return value;
+}
+
+// Exclusive scan over the block (blockDim.x a multiple of 64, at most 1024). Returns this
+// thread's exclusive prefix; *total receives the block sum. `shared` holds 16 values.
+template <typename V>
+__device__ __forceinline__ V block_exclusive_sum(V value, V* shared, V* total)
+{
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int waves = blockDim.x / kWave;
+    const V inclusive = wave_inclusive_sum(value);
+    if (lane == kWave - 1) shared[wave] = inclusive;
+    __syncthreads();
+    V before = 0;
+    V sum = 0;
+    for (int w = 0; w < waves; ++w)
+    {
+        const V s = shared[w];
+        if (w < wave) before += s;
+        sum += s;
+    }
+    __syncthreads();
+    *total = sum;
+    return before + inclusive - value;
+}
+
+template <typename V>
+__device__ __forceinline__ V block_sum(V value, V* shared)
+{
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int waves = blockDim.x / kWave;
+    value = wave_sum(value);
+    if (lane == 0) shared[wave] = value;
+    __syncthreads();
+    V sum = 0;
+    for (int w = 0; w < waves; ++w) sum += shared[w];
+    __syncthreads();
+    return sum;
+}
+
+// ---- C-ABI error plumbing -------------------------------------------------------------------
+
+void set_error(const std::string& message);
+const char* last_error();
+
+#define NDN_REQUIRE(condition, message)                                                          \
+    do                                                                                           \
+    {                                                                                            \
+        if (!(condition))                                                                        \
+        {                                                                                        \
+            ::ndn::set_error(message);                                                           \
+            return NDN_ERR_ARGUMENT;                                                             \
+        }                                                                                        \
+    } while (0)
+
+#define NDN_LAUNCHED(name)                                                                       \
+    do                                                                                           \
+    {                                                                                            \
+        const hipError_t launch_error = hipGetLastError();                                       \
+        if (launch_error != hipSuccess)                                                          \
+        {                                                                                        \
+            ::ndn::set_error(std::string(name) + ": " + hipGetErrorString(launch_error));        \
+            return NDN_ERR_HIP;                                                                  \
+        }                                                                                        \
+    } while (0)
+
+}  // namespace ndn

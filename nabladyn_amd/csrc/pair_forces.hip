@@ -1,0 +1,346 @@
+// Pair energy, forces and virial on gfx950, for LJ and tabulated potentials in fp32 and fp64.
+//
+// Two schedules share the potential functors and the exact cutoff mask:
+// - half list [2, C] (any source, the reference's NeighborPairs.index): a lane per column, the
+//   pair force added to both atoms with hardware float atomics. It serves quantity.* and the
+//   plain per-system path.
+// - full list (sliced ELL, rows of 64 in one wave): a lane per atom, forces summed in registers
+//   with no atomics and no re-reading; used by the MD step and the batched path.
+// Energy and virial are summed per block into fp64 partials, then by one block in a fixed
+// order, so they do not depend on scheduling.
+//
+// Mask and geometry follow the reference exactly: valid = (i != j) & (r^2 < rc^2), r^2 the
+// rounded (dx dx + dy dy) + dz dz of the minimum-image vector with the near-face coordinate
+// shifted first (space.py:11-19; _lennard_jones/_potential.py:218-223). Only the distance
+// test is kept free of contraction; the energy and force arithmetic may fuse.
+
+#include <algorithm>
+#include <climits>
+
+#include "common.hpp"
+#include "pair_forces.hpp"
+#include "potentials.hpp"
+
+namespace ndn {
+
+// ---- prologues ------------------------------------------------------------------------------
+
+template <typename T>
+__global__ void lj_rows_kernel(const T* __restrict__ sigma, const T* __restrict__ epsilon,
+                               int rows, T cutoff_squared, LJRow<T>* __restrict__ out)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const T s2 = sigma[row] * sigma[row];
+    const T e = epsilon[row];
+    const T base = s2 / cutoff_squared;
+    const T p6 = base * base * base;
+    out[row] = LJRow<T>{s2, T(4) * e, T(24) * e, T(4) * e * (p6 * p6 - p6)};
+}
+
+// [P, K + 2] padded knots and 1 / spacing (potential/_tabulated/_potential.py:79-82).
+template <typename T>
+__global__ void table_pad_kernel(const T* __restrict__ table, const T* __restrict__ spacing,
+                                 int rows, int knots, T* __restrict__ padded,
+                                 T* __restrict__ inverse_spacing)
+{
+    const std::int64_t index = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::int64_t width = knots + 2;
+    if (index < rows) inverse_spacing[index] = T(1) / spacing[index];
+    if (index >= rows * width) return;
+    const std::int64_t row = index / width;
+    const std::int64_t column = index % width;
+    const T* v = table + row * knots;
+    auto value = [&](std::int64_t k) { return k == knots - 1 ? T(0) : v[k]; };
+    T out;
+    if (column == 0)
+        out = T(3) * value(0) - T(3) * value(1) + value(2);
+    else if (column == width - 1)
+        out = T(3) * value(knots - 1) - T(3) * value(knots - 2) + value(knots - 3);
+    else
+        out = value(column - 1);
+    padded[index] = out;
+}
+
+template <typename T>
+__global__ void zero_kernel(T* __restrict__ data, std::int64_t count)
+{
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < count; i += stride)
+        data[i] = T(0);
+}
+
+// ---- half-list schedule ---------------------------------------------------------------------
+
+template <typename T, class Pot, bool kForces, bool kVirial>
+__global__ void __launch_bounds__(kBlock) half_list_kernel(
+    const T* __restrict__ positions, const T* __restrict__ cell,
+    const index_t* __restrict__ pairs, std::int64_t capacity, index_t num_atoms,
+    T cutoff_squared, Pot pot, T* __restrict__ forces, double* __restrict__ partials)
+{
+    __shared__ double shared[16];
+    const T lx = cell[0], ly = cell[1], lz = cell[2];
+    T energy = 0;
+    T w[6] = {0, 0, 0, 0, 0, 0};
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t column = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         column < capacity; column += stride)
+    {
+        const index_t i = pairs[column];
+        const index_t j = pairs[capacity + column];
+        // self-pairs are padding; an index outside [0, N) is skipped rather than read
+        if (i == j || static_cast<std::uint32_t>(i) >= static_cast<std::uint32_t>(num_atoms) ||
+            static_cast<std::uint32_t>(j) >= static_cast<std::uint32_t>(num_atoms))
+            continue;
+        const T* p = positions + 3LL * i;
+        const T* q = positions + 3LL * j;
+        const T dx = displacement(p[0], q[0], lx);
+        const T dy = displacement(p[1], q[1], ly);
+        const T dz = displacement(p[2], q[2], lz);
+        const T r2 = exact_squared(dx, dy, dz);
+        if (!(r2 < cutoff_squared)) continue;
+        T u, fs;
+        pot.eval(pot.row(i, j), r2, u, fs);
+        energy += u;
+        const T fx = fs * dx, fy = fs * dy, fz = fs * dz;
+        if (kForces)
+        {
+            unsafeAtomicAdd(forces + 3LL * i, fx);
+            unsafeAtomicAdd(forces + 3LL * i + 1, fy);
+            unsafeAtomicAdd(forces + 3LL * i + 2, fz);
+            unsafeAtomicAdd(forces + 3LL * j, -fx);
+            unsafeAtomicAdd(forces + 3LL * j + 1, -fy);
+            unsafeAtomicAdd(forces + 3LL * j + 2, -fz);
+        }
+        if (kVirial)
+        {
+            w[0] += dx * fx;
+            w[1] += dy * fy;
+            w[2] += dz * fz;
+            w[3] += dx * fy;
+            w[4] += dx * fz;
+            w[5] += dy * fz;
+        }
+    }
+    store_partials<kVirial>(static_cast<double>(energy), w, shared, partials);
+}
+
+// ---- final reduction ------------------------------------------------------------------------
+
+// Sums the per-block partials (8 doubles each: energy, then xx yy zz xy xz yz) in a fixed order
+// and writes E [1] and W [3, 3] (symmetric) in T, each when its pointer is given.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) finish_kernel(const double* __restrict__ partials,
+                                                        int blocks, T* energy, T* virial,
+                                                        double scale)
+{
+    __shared__ double shared[16];
+    double sums[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+        for (int k = 0; k < 7; ++k) sums[k] += partials[8LL * b + k];
+    for (int k = 0; k < 7; ++k) sums[k] = block_sum(sums[k], shared);
+    if (threadIdx.x == 0)
+    {
+        if (energy) energy[0] = static_cast<T>(scale * sums[0]);
+        if (virial)
+        {
+            const double xx = scale * sums[1], yy = scale * sums[2], zz = scale * sums[3];
+            const double xy = scale * sums[4], xz = scale * sums[5], yz = scale * sums[6];
+            virial[0] = static_cast<T>(xx);
+            virial[1] = static_cast<T>(xy);
+            virial[2] = static_cast<T>(xz);
+            virial[3] = static_cast<T>(xy);
+            virial[4] = static_cast<T>(yy);
+            virial[5] = static_cast<T>(yz);
+            virial[6] = static_cast<T>(xz);
+            virial[7] = static_cast<T>(yz);
+            virial[8] = static_cast<T>(zz);
+        }
+    }
+}
+
+// ---- host side ------------------------------------------------------------------------------
+
+int check_potential(const ndn_potential* pot, std::int64_t num_atoms)
+{
+    NDN_REQUIRE(pot != nullptr, "the potential must not be null");
+    NDN_REQUIRE(pot->num_types >= 1, "num_types must be at least 1");
+    NDN_REQUIRE(pot->num_types == 1 || pot->atom_types != nullptr,
+                "atom_types is required with more than one type");
+    NDN_REQUIRE(pot->cutoff > 0, "the cutoff must be positive");
+    if (pot->kind == NDN_LJ)
+    {
+        NDN_REQUIRE(pot->sigma && pot->epsilon, "LJ needs sigma and epsilon");
+    }
+    else
+    {
+        NDN_REQUIRE(pot->kind == NDN_TABLE, "kind must be NDN_LJ or NDN_TABLE");
+        NDN_REQUIRE(pot->table && pot->start && pot->spacing && pot->knots >= 4,
+                    "a table needs table, start, spacing and at least 4 knots");
+    }
+    (void)num_atoms;
+    return NDN_OK;
+}
+
+std::int64_t coefficient_bytes(const ndn_potential* pot, int element)
+{
+    const std::int64_t rows = static_cast<std::int64_t>(pot->num_types) * (pot->num_types + 1) / 2;
+    std::int64_t bytes;
+    if (pot->kind == NDN_LJ)
+        bytes = rows * 4 * element;
+    else
+        bytes = rows * (pot->knots + 2) * element + rows * element;
+    return (bytes + 255) / 256 * 256;
+}
+
+template <typename T>
+int prepare_potential(const ndn_potential* pot, void* workspace, hipStream_t stream,
+                      PotentialView<T>* view)
+{
+    const int rows = pot->num_types * (pot->num_types + 1) / 2;
+    const T cutoff_squared = static_cast<T>(pot->cutoff * pot->cutoff);
+    view->kind = pot->kind;
+    view->cutoff_squared = cutoff_squared;
+    if (pot->kind == NDN_LJ)
+    {
+        auto* out = static_cast<LJRow<T>*>(workspace);
+        lj_rows_kernel<T><<<blocks_for(rows), kBlock, 0, stream>>>(
+            static_cast<const T*>(pot->sigma), static_cast<const T*>(pot->epsilon), rows,
+            cutoff_squared, out);
+        NDN_LAUNCHED("lj_rows");
+        view->lj = LJ<T>{out, pot->atom_types, pot->num_types};
+    }
+    else
+    {
+        const int knots = static_cast<int>(pot->knots);
+        T* padded = static_cast<T*>(workspace);
+        T* inverse = padded + static_cast<std::int64_t>(rows) * (knots + 2);
+        table_pad_kernel<T><<<blocks_for(static_cast<std::int64_t>(rows) * (knots + 2)), kBlock, 0,
+                              stream>>>(static_cast<const T*>(pot->table),
+                                        static_cast<const T*>(pot->spacing), rows, knots, padded,
+                                        inverse);
+        NDN_LAUNCHED("table_pad");
+        view->table = Table<T>{padded, static_cast<const T*>(pot->start), inverse,
+                               pot->atom_types, pot->num_types, knots};
+    }
+    return NDN_OK;
+}
+
+template int prepare_potential<float>(const ndn_potential*, void*, hipStream_t,
+                                      PotentialView<float>*);
+template int prepare_potential<double>(const ndn_potential*, void*, hipStream_t,
+                                       PotentialView<double>*);
+
+constexpr int kHalfBlocks = 2048;
+
+int half_blocks(std::int64_t capacity)
+{
+    const std::int64_t blocks = (capacity + kBlock - 1) / kBlock;
+    return static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(blocks, kHalfBlocks)));
+}
+
+template <typename T>
+int finish(const double* partials, int blocks, void* energy, void* virial, double scale,
+           hipStream_t stream)
+{
+    if (energy == nullptr && virial == nullptr) return NDN_OK;
+    finish_kernel<T><<<1, kBlock, 0, stream>>>(partials, blocks, static_cast<T*>(energy),
+                                                static_cast<T*>(virial), scale);
+    NDN_LAUNCHED("finish");
+    return NDN_OK;
+}
+
+template int finish<float>(const double*, int, void*, void*, double, hipStream_t);
+template int finish<double>(const double*, int, void*, void*, double, hipStream_t);
+
+template <typename T, class Pot>
+void launch_half(const T* x, const T* cell, const index_t* pairs, std::int64_t capacity,
+                 index_t num_atoms, T cutoff_squared, const Pot& pot, T* forces, bool virial,
+                 double* partials, int blocks, hipStream_t stream)
+{
+    if (forces && virial)
+        half_list_kernel<T, Pot, true, true><<<blocks, kBlock, 0, stream>>>(
+            x, cell, pairs, capacity, num_atoms, cutoff_squared, pot, forces, partials);
+    else if (forces)
+        half_list_kernel<T, Pot, true, false><<<blocks, kBlock, 0, stream>>>(
+            x, cell, pairs, capacity, num_atoms, cutoff_squared, pot, forces, partials);
+    else if (virial)
+        half_list_kernel<T, Pot, false, true><<<blocks, kBlock, 0, stream>>>(
+            x, cell, pairs, capacity, num_atoms, cutoff_squared, pot, forces, partials);
+    else
+        half_list_kernel<T, Pot, false, false><<<blocks, kBlock, 0, stream>>>(
+            x, cell, pairs, capacity, num_atoms, cutoff_squared, pot, forces, partials);
+}
+
+template <typename T>
+int pair_efw_impl(const void* positions, const void* cell, std::int64_t num_atoms,
+                  const index_t* pairs, std::int64_t capacity, const ndn_potential* pot,
+                  void* energy, void* forces, void* virial, void* workspace, hipStream_t stream)
+{
+    PotentialView<T> view;
+    const int prepared = prepare_potential<T>(pot, workspace, stream, &view);
+    if (prepared != NDN_OK) return prepared;
+    const int blocks = half_blocks(capacity);
+    auto* partials = reinterpret_cast<double*>(static_cast<std::uint8_t*>(workspace) +
+                                               coefficient_bytes(pot, sizeof(T)));
+    T* f = static_cast<T*>(forces);
+    if (f != nullptr)
+    {
+        zero_kernel<T><<<std::min<unsigned>(blocks_for(3 * num_atoms), 1024u), kBlock, 0,
+                         stream>>>(f, 3 * num_atoms);
+        NDN_LAUNCHED("zero_forces");
+    }
+    const T* x = static_cast<const T*>(positions);
+    const T* L = static_cast<const T*>(cell);
+    if (view.kind == NDN_LJ)
+        launch_half<T>(x, L, pairs, capacity, static_cast<index_t>(num_atoms), view.cutoff_squared,
+                       view.lj, f, virial != nullptr, partials, blocks, stream);
+    else
+        launch_half<T>(x, L, pairs, capacity, static_cast<index_t>(num_atoms), view.cutoff_squared,
+                       view.table, f, virial != nullptr, partials, blocks, stream);
+    NDN_LAUNCHED("half_list");
+    return finish<T>(partials, blocks, energy, virial, 1.0, stream);
+}
+
+}  // namespace ndn
+
+using namespace ndn;
+
+extern "C" {
+
+int ndn_pair_workspace_bytes(int dtype, const ndn_potential* pot, int64_t columns,
+                             int64_t* bytes)
+{
+    const int checked = check_potential(pot, 0);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    NDN_REQUIRE(columns >= 0 && bytes != nullptr, "columns must be nonnegative");
+    const int element = dtype == NDN_F32 ? 4 : 8;
+    const std::int64_t blocks = std::max<std::int64_t>(half_blocks(columns), kPartialSlots);
+    *bytes = coefficient_bytes(pot, element) + blocks * 8 * static_cast<std::int64_t>(sizeof(double));
+    return NDN_OK;
+}
+
+int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                 const int32_t* pairs, int64_t capacity, const ndn_potential* pot, void* energy,
+                 void* forces, void* virial, void* workspace, int64_t workspace_bytes,
+                 ndn_stream_t stream)
+{
+    const int checked = check_potential(pot, num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(positions && cell && pairs && workspace, "a buffer is null");
+    NDN_REQUIRE(num_atoms >= 1 && num_atoms <= INT32_MAX, "N must satisfy 1 <= N < 2**31");
+    NDN_REQUIRE(capacity >= 1 && capacity <= INT32_MAX, "pairs must be [2, C] with int32 C");
+    std::int64_t needed = 0;
+    const int sized = ndn_pair_workspace_bytes(dtype, pot, capacity, &needed);
+    if (sized != NDN_OK) return sized;
+    NDN_REQUIRE(workspace_bytes >= needed, "the workspace is too small");
+    if (dtype == NDN_F32)
+        return pair_efw_impl<float>(positions, cell, num_atoms, pairs, capacity, pot, energy,
+                                    forces, virial, workspace, stream);
+    return pair_efw_impl<double>(positions, cell, num_atoms, pairs, capacity, pot, energy, forces,
+                                 virial, workspace, stream);
+}
+
+}  // extern "C"

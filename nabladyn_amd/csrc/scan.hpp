@@ -1,0 +1,129 @@
+// A gated exclusive scan of int32 values for the list builds: three launches (tile sums, one
+// block scanning the tile sums, tiles applied), each of which returns at once when the build
+// is not running, so a captured update pays three empty launches on steps without a rebuild.
+// The tile sums live after the list header in the caller's scan_temp buffer.
+#pragma once
+
+#include "common.hpp"
+
+namespace ndn {
+
+constexpr int kScanPerThread = 4;
+constexpr int kScanTile = kBlock * kScanPerThread;  // 1024 values per block
+
+// Bytes of the list header at the start of scan_temp; the tile sums follow it.
+constexpr std::int64_t kHeaderBytes = 256;
+
+struct ListHeader
+{
+    index_t old_total;                 // pair_total before this build: columns past it hold padding
+    index_t pad0;
+    unsigned long long max_moved_bits; // largest squared displacement at the last reuse check
+};
+
+inline std::int64_t scan_tiles(std::int64_t items)
+{
+    return (items + kScanTile - 1) / kScanTile;
+}
+
+inline std::int64_t scan_bytes(std::int64_t items)
+{
+    return kHeaderBytes + 4 * (scan_tiles(items) + 1);
+}
+
+__global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
+    const index_t* __restrict__ input, std::int64_t count, index_t* __restrict__ tile_sums,
+    const index_t* rebuild, const index_t* status)
+{
+    if (!rebuilding(rebuild, status)) return;
+    __shared__ index_t shared[16];
+    const std::int64_t base = static_cast<std::int64_t>(blockIdx.x) * kScanTile;
+    index_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+    {
+        const std::int64_t i = base + threadIdx.x + static_cast<std::int64_t>(k) * kBlock;
+        if (i < count) local += input[i];
+    }
+    const index_t sum = block_sum(local, shared);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = sum;
+}
+
+__global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
+    index_t* __restrict__ tile_sums, std::int64_t tiles, const index_t* rebuild,
+    const index_t* status)
+{
+    if (!rebuilding(rebuild, status)) return;
+    __shared__ index_t shared[16];
+    index_t carry = 0;
+    for (std::int64_t base = 0; base < tiles; base += kScanTile)
+    {
+        index_t v[kScanPerThread];
+        index_t local = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k)
+        {
+            const std::int64_t i = base + threadIdx.x * kScanPerThread + k;
+            v[k] = i < tiles ? tile_sums[i] : 0;
+            local += v[k];
+        }
+        index_t total;
+        index_t running = carry + block_exclusive_sum(local, shared, &total);
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k)
+        {
+            const std::int64_t i = base + threadIdx.x * kScanPerThread + k;
+            if (i < tiles) tile_sums[i] = running;
+            running += v[k];
+        }
+        carry += total;
+        __syncthreads();
+    }
+}
+
+// out may alias in: each block reads its whole tile before writing it.
+__global__ void __launch_bounds__(kBlock) scan_apply_kernel(
+    const index_t* input, std::int64_t count, const index_t* __restrict__ tile_prefix,
+    index_t* output, const index_t* rebuild, const index_t* status)
+{
+    if (!rebuilding(rebuild, status)) return;
+    __shared__ index_t shared[16];
+    const std::int64_t base =
+        static_cast<std::int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanPerThread;
+    index_t v[kScanPerThread];
+    index_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+    {
+        v[k] = base + k < count ? input[base + k] : 0;
+        local += v[k];
+    }
+    index_t total;
+    index_t running = tile_prefix[blockIdx.x] + block_exclusive_sum(local, shared, &total);
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+    {
+        if (base + k < count) output[base + k] = running;
+        running += v[k];
+    }
+}
+
+// Exclusive scan of `count` values; returns NDN_OK or NDN_ERR_HIP.
+inline int exclusive_scan(const index_t* input, index_t* output, std::int64_t count,
+                          std::uint8_t* scan_temp, const index_t* rebuild, const index_t* status,
+                          hipStream_t stream)
+{
+    const std::int64_t tiles = scan_tiles(count);
+    index_t* tile_sums = reinterpret_cast<index_t*>(scan_temp + kHeaderBytes);
+    scan_tile_sums_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        input, count, tile_sums, rebuild, status);
+    NDN_LAUNCHED("scan_tile_sums");
+    scan_tile_prefix_kernel<<<1, kBlock, 0, stream>>>(tile_sums, tiles, rebuild, status);
+    NDN_LAUNCHED("scan_tile_prefix");
+    scan_apply_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        input, count, tile_sums, output, rebuild, status);
+    NDN_LAUNCHED("scan_apply");
+    return NDN_OK;
+}
+
+}  // namespace ndn

@@ -1,0 +1,61 @@
+"""The C-ABI library loads and exports every symbol its header declares; the torch ops carry
+the reference's schemas. No GPU compute here."""
+
+import ctypes
+
+import pytest
+import torch
+
+from nabladyn_amd import _native
+from nabladyn_amd.neighbors import NeighborList, _ops
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.lib()
+    declared = _native.exported_symbols()
+    assert len(declared) >= 8
+    missing = [name for name in declared if not hasattr(lib, name)]
+    assert missing == []
+
+
+def test_scan_temp_bytes_and_argument_errors():
+    out = ctypes.c_int64()
+    _native.call("scan_temp_bytes", ctypes.c_int64(1000), ctypes.byref(out))
+    assert out.value >= 256 + 4
+    with pytest.raises(_native.NativeError, match="fit int32"):
+        _native.call("scan_temp_bytes", ctypes.c_int64(0), ctypes.byref(out))
+    buffers = _native.NeighborBuffers()
+    with pytest.raises(_native.NativeError, match="N must satisfy"):
+        _native.call("count_first_frame", 0, None, None, ctypes.c_double(1.0),
+                     ctypes.byref(buffers), None)
+
+
+def test_op_schemas_match_the_reference():
+    ops = _ops.ops()
+    reference = {
+        "scan_temp_bytes": "nabladyn_neighbors::scan_temp_bytes(Tensor reference, int items) -> int",
+    }
+    schema = str(ops.scan_temp_bytes.default._schema)
+    assert schema == reference["scan_temp_bytes"]
+    update = str(ops.update.default._schema)
+    for name in ("Tensor(o!) reference_positions", "Tensor(t!) bits", "float cutoff, float radius",
+                 "Tensor limits", "Tensor(q!) reuse"):
+        assert name in update
+    assert ops.scan_temp_bytes(torch.empty(1), 100) > 0
+
+
+def test_neighbor_list_validation_on_host():
+    with pytest.raises(ValueError):
+        NeighborList(0.0)
+    with pytest.raises(ValueError):
+        NeighborList(1.0, skin=-0.1)
+    with pytest.raises(ValueError):
+        NeighborList(1.0, capacity_multiplier=0.5)
+    with pytest.raises(ValueError):
+        NeighborList(1.0, inner_cutoffs=(1.5,))
+    nl = NeighborList(1.0, skin=0.1)
+    assert nl.radius == pytest.approx(1.1)
+    with pytest.raises(TypeError):
+        nl.allocate(torch.zeros(4, 3), torch.ones(3) * 3)
+    with pytest.raises(RuntimeError, match="allocate"):
+        nl.tensors()
