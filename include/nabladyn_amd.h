@@ -132,6 +132,129 @@ int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num
                  void* forces, void* virial, void* workspace, int64_t workspace_bytes,
                  ndn_stream_t stream);
 
+/* ---- molecular dynamics ----------------------------------------------------------------- */
+
+/* The resident frame of a simulation (md/_state.py:8-21), kept in cell order ("slots"): every
+ * per-atom array below is permuted at each rebuild, and ndn_md_list.ids maps a slot to its atom.
+ * thermostat [8] f64: chain positions [3], momenta [3], the closing velocity factor still to be
+ * applied (1 when settled), the current opening factor. kinetic [1] f64 is the K the chain
+ * reads (md/_ensembles/_nvt.py:47-49). partials [partial_slots * 8] f64 and coefficients
+ * [coefficient_bytes] are scratch sized by ndn_md_workspace. virial may be NULL. */
+typedef struct ndn_md_system {
+    int64_t num_atoms;
+    void* positions;            /* [N, 3] T, wrapped */
+    void* velocities;           /* [N, 3] T */
+    void* forces;               /* [N, 3] T */
+    int64_t* images;            /* [N, 3] */
+    void* masses;               /* [N] T */
+    void* inverse_masses;       /* [N] T, 1 / m */
+    const void* cell;           /* [3] T */
+    void* energy;               /* [1] T */
+    void* virial;               /* [3, 3] T or NULL */
+    double* thermostat;         /* [8] */
+    const double* chain_masses; /* [3]: g kT tau^2, kT tau^2, kT tau^2 */
+    double* kinetic;            /* [1] */
+    double* partials;
+    int64_t partial_slots;
+    void* coefficients;
+    int64_t coefficient_bytes;
+} ndn_md_system;
+
+typedef struct ndn_md_settings {
+    double timestep;              /* ps */
+    double kT;                    /* kJ/mol (NVT) */
+    int64_t degrees_of_freedom;   /* 3N - 3 */
+    int thermostat;               /* 0: NVE, 1: NVT with a Nose-Hoover chain of 3 */
+    int virial;                   /* also compute W each step */
+} ndn_md_settings;
+
+/* The reuse state of a neighbor list (neighbors/_neighbor_list.py:42-56). */
+typedef struct ndn_list_state {
+    double cutoff;
+    double radius;
+    void* reference_positions;    /* [N, 3] T, slot order */
+    void* reference_cell;         /* [3] T */
+    const int32_t* limits;        /* [3] */
+    void* reuse;                  /* [4] T */
+    int32_t* rebuild;             /* [1] */
+    int32_t* rebuild_count;       /* [1] */
+} ndn_list_state;
+
+/* The MD step's own neighbor structure: sub-bins at least radius / 2 wide (dims fixed for the
+ * run), and a full list (both orientations of every pair within the radius) in fixed-stride rows,
+ * one per slot, holding neighbor slots. `group` lanes of a wave read one row in the force kernel
+ * (1, 4, 8, 16 or 32). status [2] latches errors as the neighbor list's does, with
+ * NDN_STATUS_FULL_OVERFLOW when a row needs more than `stride` entries (status[1] = the width
+ * needed). scan_temp holds a 1024-byte header and the scan tiles. */
+typedef struct ndn_md_list {
+    int32_t dims[3];
+    int32_t group;
+    double widths[3];
+    int64_t stride;
+    int32_t* bin_counts;        /* [bins + 1], zero between builds */
+    int32_t* bin_offsets;       /* [bins + 1] */
+    int32_t* bin_of;            /* [N] */
+    int32_t* unordered;         /* [N] */
+    int32_t* order;             /* [N] new slot -> previous slot */
+    int32_t* counts;            /* [N] neighbors of each slot */
+    int32_t* index;             /* [N * stride] neighbor slots, row-major; NULL while stride is 0 */
+    int32_t* ids;               /* [N] atom id of each slot */
+    int64_t* types;             /* [N] atom type of each slot, or NULL with one type */
+    void* positions4;           /* [N, 4] T */
+    void* scratch;              /* [ndn_md_scratch_bytes] */
+    uint8_t* scan_temp;
+    int64_t scan_temp_bytes;
+    int32_t* status;            /* [2] */
+    int32_t* prepare_dims;      /* [3] scratch of the reuse rule */
+    void* prepare_widths;       /* [3] T scratch */
+    int32_t* prepare_total;     /* [1] scratch */
+} ndn_md_list;
+
+/* Scratch sizes of a system: partial slots and potential-coefficient bytes. */
+int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
+                     int64_t* partial_slots, int64_t* coefficient_bytes);
+
+/* Bytes of ndn_md_list.scratch for N atoms. */
+int64_t ndn_md_scratch_bytes(int dtype, int64_t num_atoms);
+
+/* The list build of the MD path: sort into sub-bins, permute the frame, fill the rows (count
+ * only while stride is 0), store the reference frame. With `forced` it always runs; otherwise
+ * it is gated on list->rebuild as inside a step. */
+int ndn_md_rebuild(int dtype, const ndn_md_system* system, const ndn_list_state* list,
+                   const ndn_md_list* md_list, int forced, ndn_stream_t stream);
+
+/* One MD step (NVE or NVT), launched on the stream with no sync or allocation, so a HIP graph
+ * can capture it: chain half step, first half kick + drift + wrap, the reuse check and the gated
+ * rebuild, forces over the full list with the second half kick and K, then E and the chain's
+ * closing half step. Replaces one replay of the reference's captured step
+ * (md/_simulation.py:316-364: first_half, NeighborList._update_wrapped, second_half, closing). */
+int ndn_md_step(int dtype, const ndn_md_system* system, const ndn_md_settings* settings,
+                const ndn_potential* pot, const ndn_list_state* list, const ndn_md_list* md_list,
+                ndn_stream_t stream);
+
+/* E (and W when `virial`) and F of the current frame over the full list; with `kick` also the
+ * second half kick of `timestep`. The first frame, the sampled panel (md/_simulation.py:177-180,
+ * 366-402), and the force launch alone for timing. */
+int ndn_md_evaluate(int dtype, const ndn_md_system* system, const ndn_potential* pot,
+                    const ndn_md_list* md_list, int virial, int kick, double timestep,
+                    ndn_stream_t stream);
+
+/* The force launch alone (the step's dominant kernel), on potential coefficients a step or an
+ * evaluate already prepared, so events can time it; with `kick` it also advances the
+ * velocities by the second half kick. No reference counterpart. */
+int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* pot,
+                 const ndn_md_list* md_list, int virial, int kick, double timestep,
+                 ndn_stream_t stream);
+
+/* Applies the pending closing velocity factor, leaving the velocities as the reference's frame
+ * holds them after a step. */
+int ndn_md_settle(int dtype, const ndn_md_system* system, ndn_stream_t stream);
+
+/* Slot order -> atom order: out[ids[s] * width + k] = in[s * width + k], elements of 4 or 8
+ * bytes. */
+int ndn_md_to_atom_order(int element_bytes, const void* in, const int32_t* ids, int64_t num_atoms,
+                         int width, void* out, ndn_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,6 @@ Drop-in modules: ``neighbors`` (NeighborList, NeighborPairs, the ``nabladyn_neig
 ``serve`` (continuous-batching request loop), ``fit`` (hand-written backward).
 """
 
-from . import neighbors, observables, potential, quantity, space, units
+from . import md, neighbors, observables, potential, quantity, space, units
 
-__all__ = ("neighbors", "observables", "potential", "quantity", "space", "units")
+__all__ = ("md", "neighbors", "observables", "potential", "quantity", "space", "units")

@@ -1,0 +1,1203 @@
+// The molecular-dynamics step on gfx950: velocity Verlet with an optional Nose-Hoover chain of
+// three, the neighbor update and the pair forces, replayed as one HIP graph per step.
+//
+// The resident frame is kept in cell order ("slots"): at every rebuild the atoms are counting-
+// sorted into sub-bins at least half the list radius wide, ordered by atom id inside a sub-bin
+// (so every run is deterministic), and the whole frame is permuted with them. Every per-atom
+// access of a step is then coalesced, and a neighbor's position is a short gather from nearby
+// memory. The frame is read out in atom order only on demand.
+//
+//   prologue     one block: the chain's opening half step (fp64), the reuse rule's preparation
+//                (update/update.cuh:27-75), the potential's coefficient rows
+//   first_half   per slot: chain scale, half kick, drift, wrap, image count, the float4 copy the
+//                force kernel gathers from, and the squared displacement since the last rebuild
+//                reduced to the rebuild flag (update/update.cuh:79-116)
+//   rebuild      gated on that flag: sub-bin counting sort, permutation of the frame, then one
+//                wave per atom tests the candidates of its culled 5 x 5 x 5 sub-bin stencil 64 at
+//                a time and ballots the hits into its row of a fixed-stride full list
+//   force        a lane group per row reads its row coalesced, sums the force in registers, then
+//                the second half kick and the partial sums of E, W and K
+//   finish       one block: E, W, K in fixed order and the chain's closing half step; its
+//                velocity factor is applied by the next first_half (or by settle)
+//
+// Reference step order (md/_simulation.py:343-348): first_half, NeighborList._update_wrapped,
+// energy_and_force + second_half, closing; ensembles in md/_ensembles/_verlet.py:8-31,
+// _nve.py:39-58, _nvt.py:88-129, _chain.py:29-105. The integrator's arithmetic is the
+// reference's eager expressions, each product and sum rounded on its own; the list's distance
+// test is the exact minimum-image expression, so its pair set equals the reference's.
+
+#include <algorithm>
+#include <climits>
+
+#include "common.hpp"
+#include "neighbors.hpp"
+#include "pair_forces.hpp"
+#include "potentials.hpp"
+
+namespace ndn {
+
+// Thermostat slots (fp64): chain positions [0..2], momenta [3..5], the pending closing factor
+// [6], the opening factor of the current step [7].
+constexpr int kChainQ = 0;
+constexpr int kChainP = 3;
+constexpr int kPending = 6;
+constexpr int kOpening = 7;
+constexpr int kStencil = 5;          // sub-bins per axis a stencil spans
+
+// The chain's half step against the kinetic energy K it thermostats (_chain.py:29-76), in the
+// reference's operation order. Updates q, p and K; returns the velocity factor.
+__device__ __forceinline__ double chain_half_step(double* kinetic, double* q, double* p,
+                                                  const double* mass, double g, double kT,
+                                                  double dt)
+{
+#pragma clang fp contract(off)
+    const double quarter = 0.25 * dt;
+    const double eighth = 0.125 * dt;
+    double damping[2];
+    double K = *kinetic;
+    auto driving = [&](int index) {
+        return index == 0 ? 2.0 * K - g * kT : p[index - 1] * p[index - 1] / mass[index - 1] - kT;
+    };
+    for (int index = 2; index >= 0; --index)
+    {
+        if (index == 2)
+            p[2] = p[2] + quarter * driving(2);
+        else
+        {
+            const double kick = quarter * driving(index);
+            p[index] = damping[index] * (damping[index] * p[index] + kick);
+        }
+        if (index > 0) damping[index - 1] = exp(-eighth * p[index] / mass[index]);
+    }
+    const double factor = exp(-0.5 * dt * p[0] / mass[0]);
+    K = K * (factor * factor);
+    for (int k = 0; k < 3; ++k) q[k] = q[k] + 0.5 * dt * p[k] / mass[k];
+    for (int index = 0; index <= 2; ++index)
+    {
+        if (index == 2)
+            p[2] = p[2] + quarter * driving(2);
+        else
+        {
+            const double kick = quarter * driving(index);
+            p[index] = damping[index] * (damping[index] * p[index] + kick);
+        }
+    }
+    *kinetic = K;
+    return factor;
+}
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float>
+{
+    using type = float4;
+};
+template <>
+struct Vec4<double>
+{
+    using type = double4;
+};
+
+template <typename T>
+__device__ __forceinline__ T reciprocal(T value);
+template <>
+__device__ __forceinline__ float reciprocal<float>(float value)
+{
+    return __builtin_amdgcn_rcpf(value);
+}
+template <>
+__device__ __forceinline__ double reciprocal<double>(double value)
+{
+    return 1.0 / value;
+}
+
+// LJ with the hardware reciprocal (1 ulp in fp32): the force kernel's potential.
+template <typename T>
+struct FastLJ
+{
+    const LJRow<T>* __restrict__ rows;
+    const std::int64_t* __restrict__ types;
+    int num_types;
+
+    __device__ __forceinline__ int row(std::int64_t i, std::int64_t j) const
+    {
+        return num_types == 1 ? 0 : pair_row(types[i], types[j], num_types);
+    }
+
+    __device__ __forceinline__ void eval(int row_index, T r2, T& u, T& fs) const
+    {
+        const LJRow<T> c = rows[row_index];
+        const T inverse = reciprocal(r2);
+        const T s = c.sigma_squared * inverse;
+        const T p6 = s * s * s;
+        u = c.four_eps * (p6 * p6 - p6) - c.shift;
+        fs = c.twentyfour_eps * (T(2) * p6 * p6 - p6) * inverse;
+    }
+};
+
+template <typename T>
+struct StepArgs
+{
+    // frame, in slot order
+    T* positions;
+    T* velocities;
+    T* forces;
+    std::int64_t* images;
+    T* masses;
+    T* inverse_masses;
+    const T* cell;
+    T* energy;
+    T* virial;
+    index_t num_atoms;
+    // thermostat
+    double* thermostat;
+    const double* chain_masses;
+    double* kinetic;
+    double kT;
+    double degrees;
+    double timestep;
+    int nvt;
+    // reuse rule
+    T cutoff;
+    T radius;
+    T* reference_positions;
+    T* reference_cell;
+    const index_t* limits;
+    T* reuse;
+    index_t* rebuild;
+    index_t* rebuild_count;
+    // list
+    index_t* status;
+    index_t* dims_scratch;
+    T* widths_scratch;
+    index_t* total_scratch;
+    ListHeader* header;
+    index_t nx, ny, nz;
+    T wx, wy, wz;
+    std::int64_t bins;
+    std::int64_t stride;
+    index_t* bin_counts;
+    index_t* bin_offsets;
+    index_t* bin_of;
+    index_t* unordered;
+    index_t* order;
+    index_t* counts;
+    index_t* index;
+    index_t* ids;
+    std::int64_t* types;
+    T* positions4;
+    std::uint8_t* scratch;
+    std::uint8_t* scan_temp;
+};
+
+// ---- prologue -------------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) prologue_kernel(StepArgs<T> a, const ndn_potential pot,
+                                                          void* coefficients)
+{
+    if (threadIdx.x == 0)
+    {
+        if (a.nvt)
+        {
+            double* t = a.thermostat;
+            t[kOpening] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
+                                          a.degrees, a.kT, a.timestep);
+        }
+        prepare_update(a.cell, a.reference_cell, a.cutoff, a.radius, a.limits, a.dims_scratch,
+                       a.widths_scratch, a.reuse, a.rebuild, a.status, a.total_scratch, a.header);
+    }
+    const int rows = pot.num_types * (pot.num_types + 1) / 2;
+    if (pot.kind == NDN_LJ)
+    {
+        auto* out = static_cast<LJRow<T>*>(coefficients);
+        const T* sigma = static_cast<const T*>(pot.sigma);
+        const T* epsilon = static_cast<const T*>(pot.epsilon);
+        const T cutoff_squared = static_cast<T>(pot.cutoff * pot.cutoff);
+        for (int row = threadIdx.x; row < rows; row += blockDim.x)
+        {
+            const T s2 = sigma[row] * sigma[row];
+            const T e = epsilon[row];
+            const T base = s2 / cutoff_squared;
+            const T p6 = base * base * base;
+            out[row] = LJRow<T>{s2, T(4) * e, T(24) * e, T(4) * e * (p6 * p6 - p6)};
+        }
+    }
+    else
+    {
+        const int knots = static_cast<int>(pot.knots);
+        const std::int64_t width = knots + 2;
+        T* padded = static_cast<T*>(coefficients);
+        T* inverse = padded + rows * width;
+        const T* table = static_cast<const T*>(pot.table);
+        const T* spacing = static_cast<const T*>(pot.spacing);
+        for (std::int64_t i = threadIdx.x; i < rows * width; i += blockDim.x)
+        {
+            const std::int64_t row = i / width;
+            const std::int64_t column = i % width;
+            const T* v = table + row * knots;
+            auto value = [&](std::int64_t k) { return k == knots - 1 ? T(0) : v[k]; };
+            T out;
+            if (column == 0)
+                out = T(3) * value(0) - T(3) * value(1) + value(2);
+            else if (column == width - 1)
+                out = T(3) * value(knots - 1) - T(3) * value(knots - 2) + value(knots - 3);
+            else
+                out = value(column - 1);
+            padded[i] = out;
+        }
+        for (int row = threadIdx.x; row < rows; row += blockDim.x) inverse[row] = T(1) / spacing[row];
+    }
+}
+
+// ---- first half -----------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T timestep,
+                                                            T half_timestep)
+{
+#pragma clang fp contract(off)
+    using V4 = typename Vec4<T>::type;
+    __shared__ T block_max[kBlock / kWave];
+    const std::int64_t atom = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool check = a.status[0] == 0;
+    bool finite = true;
+    T moved = 0;
+    if (atom < a.num_atoms)
+    {
+        T* v = a.velocities + 3 * atom;
+        const T* f = a.forces + 3 * atom;
+        T* x = a.positions + 3 * atom;
+        const T inverse_mass = a.inverse_masses[atom];
+        T vx = v[0], vy = v[1], vz = v[2];
+        if (a.nvt)
+        {
+            // the previous step's closing factor, then this step's opening factor
+            const T closing = static_cast<T>(a.thermostat[kPending]);
+            const T opening = static_cast<T>(a.thermostat[kOpening]);
+            vx = vx * closing;
+            vy = vy * closing;
+            vz = vz * closing;
+            vx = vx * opening;
+            vy = vy * opening;
+            vz = vz * opening;
+        }
+        vx = vx + half_timestep * f[0] * inverse_mass;
+        vy = vy + half_timestep * f[1] * inverse_mass;
+        vz = vz + half_timestep * f[2] * inverse_mass;
+        T out[3];
+        const T vel[3] = {vx, vy, vz};
+        std::int64_t crossings[3];
+        bool crossed = false;
+        for (int axis = 0; axis < 3; ++axis)
+        {
+            const T length = a.cell[axis];
+            const T drifted = x[axis] + timestep * vel[axis];
+            T k = floor(drifted / length);
+            T w = drifted - k * length;
+            if (w < T(0))
+            {
+                w = w + length;
+                k = k - T(1);
+            }
+            if (w >= length)
+            {
+                w = w - length;
+                k = k + T(1);
+            }
+            out[axis] = w;
+            crossings[axis] = static_cast<std::int64_t>(k);
+            crossed = crossed || crossings[axis] != 0;
+        }
+        v[0] = vx;
+        v[1] = vy;
+        v[2] = vz;
+        x[0] = out[0];
+        x[1] = out[1];
+        x[2] = out[2];
+        V4 p4;
+        p4.x = out[0];
+        p4.y = out[1];
+        p4.z = out[2];
+        p4.w = T(0);
+        reinterpret_cast<V4*>(a.positions4)[atom] = p4;
+        if (crossed)
+        {
+            std::int64_t* image = a.images + 3 * atom;
+            image[0] += crossings[0];
+            image[1] += crossings[1];
+            image[2] += crossings[2];
+        }
+        if (check)
+        {
+            finite = isfinite(out[0]) && isfinite(out[1]) && isfinite(out[2]);
+            if (finite)
+                moved = moved_squared(out[0], out[1], out[2], a.reference_positions + 3 * atom,
+                                      a.reuse, a.cell);
+        }
+    }
+    // every thread reaches the barrier below, whatever status a wave read
+    if (!finite) atomicOr(a.status, kStatusNonFinite);
+    const T wave_moved = wave_max(finite ? moved : T(0));
+    const int wave = threadIdx.x / kWave;
+    if (lane_id() == 0)
+    {
+        block_max[wave] = wave_moved;
+        if (check && wave_moved >= a.reuse[0] && a.rebuild[0] == 0) a.rebuild[0] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        T m = block_max[0];
+        for (int w = 1; w < kBlock / kWave; ++w) m = block_max[w] > m ? block_max[w] : m;
+        if (m > T(0)) atomicMax(moved_slot(a.header, blockIdx.x), order_bits(m));
+    }
+}
+
+// ---- rebuild --------------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ index_t sub_bin(T x, T width, index_t count)
+{
+    const index_t bin = static_cast<index_t>(floor(x / width));
+    return bin < 0 ? 0 : (bin >= count ? count - 1 : bin);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_assign_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= a.num_atoms) return;
+    const T* p = a.positions + 3 * slot;
+    const index_t bin = sub_bin(p[0], a.wx, a.nx) +
+                        a.nx * (sub_bin(p[1], a.wy, a.ny) + a.ny * sub_bin(p[2], a.wz, a.nz));
+    a.bin_of[slot] = bin;
+    atomicAdd(a.bin_counts + bin, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_scatter_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= a.num_atoms) return;
+    const index_t bin = a.bin_of[slot];
+    a.unordered[a.bin_offsets[bin] + atomicSub(a.bin_counts + bin, 1) - 1] =
+        static_cast<index_t>(slot);
+}
+
+// One thread per sub-bin (a handful of atoms): ranks each entry by atom id, so the new order is
+// (sub-bin, atom id) whatever the atomics did.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_order_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t bin = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (bin >= a.bins) return;
+    const index_t begin = a.bin_offsets[bin];
+    const index_t end = a.bin_offsets[bin + 1];
+    for (index_t i = begin; i < end; ++i)
+    {
+        const index_t slot = a.unordered[i];
+        const index_t id = a.ids[slot];
+        index_t rank = 0;
+        for (index_t k = begin; k < end; ++k) rank += a.ids[a.unordered[k]] < id;
+        a.order[begin + rank] = slot;
+    }
+}
+
+// One atom of the frame, as the permutation moves it.
+template <typename T>
+struct Packed
+{
+    T x[3], v[3], f[3];
+    std::int64_t image[3];
+    T mass, inverse_mass;
+    std::int64_t type;
+    index_t id;
+};
+
+// Gathers the frame into the new order in scratch; md_unpack_kernel puts it home.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_permute_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= a.num_atoms) return;
+    const std::int64_t old = a.order[slot];
+    Packed<T> p;
+    for (int k = 0; k < 3; ++k)
+    {
+        p.x[k] = a.positions[3 * old + k];
+        p.v[k] = a.velocities[3 * old + k];
+        p.f[k] = a.forces[3 * old + k];
+        p.image[k] = a.images[3 * old + k];
+    }
+    p.mass = a.masses[old];
+    p.inverse_mass = a.inverse_masses[old];
+    p.type = a.types ? a.types[old] : 0;
+    p.id = a.ids[old];
+    reinterpret_cast<Packed<T>*>(a.scratch)[slot] = p;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int forced)
+{
+    using V4 = typename Vec4<T>::type;
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= a.num_atoms) return;
+    const Packed<T> p = reinterpret_cast<const Packed<T>*>(a.scratch)[slot];
+    for (int k = 0; k < 3; ++k)
+    {
+        a.positions[3 * slot + k] = p.x[k];
+        a.velocities[3 * slot + k] = p.v[k];
+        a.forces[3 * slot + k] = p.f[k];
+        a.images[3 * slot + k] = p.image[k];
+    }
+    a.masses[slot] = p.mass;
+    a.inverse_masses[slot] = p.inverse_mass;
+    if (a.types) a.types[slot] = p.type;
+    a.ids[slot] = p.id;
+    V4 p4;
+    p4.x = p.x[0];
+    p4.y = p.x[1];
+    p4.z = p.x[2];
+    p4.w = T(0);
+    reinterpret_cast<V4*>(a.positions4)[slot] = p4;
+}
+
+// Distance from coordinate c to the slab [lo, lo + w): 0 inside.
+template <typename T>
+__device__ __forceinline__ T slab_distance(T c, T lo, T w)
+{
+    const T below = lo - c;
+    const T above = c - (lo + w);
+    return below > T(0) ? below : (above > T(0) ? above : T(0));
+}
+
+// One wave per atom (slot): collects the contiguous slot ranges of the sub-bins its stencil
+// reaches (runs of x-consecutive sub-bins, culled by distance), packs them 64 candidates at a
+// time, tests each with the exact minimum-image expression and ballots the hits into the atom's
+// row. With stride 0 it only counts (the first frame, to size the rows).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius_squared,
+                                                         T cull_squared, int forced)
+{
+#pragma clang fp contract(off)
+    using V4 = typename Vec4<T>::type;
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t center =
+        static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+    if (center >= a.num_atoms) return;
+    const int lane = lane_id();
+    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
+    const V4 c = pos[center];
+    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
+    const index_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const index_t hx = sub_bin(c.x, a.wx, nx), hy = sub_bin(c.y, a.wy, ny),
+                  hz = sub_bin(c.z, a.wz, nz);
+    const bool full_x = nx < kStencil, full_y = ny < kStencil, full_z = nz < kStencil;
+    const int span_x = full_x ? nx : kStencil;
+    const int span_y = full_y ? ny : kStencil;
+    const int span_z = full_z ? nz : kStencil;
+    // this lane's range: stencil row lane / 2, part lane % 2 of its run of x sub-bins
+    index_t lo = 0, length = 0;
+    const int row = lane / 2, part = lane % 2;
+    if (row < span_y * span_z)
+    {
+        const int oy = row % span_y, oz = row / span_y;
+        const index_t by = full_y ? oy : hy + oy - 2;  // unwrapped, for the distance
+        const index_t bz = full_z ? oz : hz + oz - 2;
+        const T dy = full_y ? T(0) : slab_distance(c.y, by * a.wy, a.wy);
+        const T dz = full_z ? T(0) : slab_distance(c.z, bz * a.wz, a.wz);
+        const T dyz = dy * dy + dz * dz;
+        if (dyz < cull_squared)
+        {
+            int first = span_x, last = -1;
+            for (int ox = 0; ox < span_x; ++ox)
+            {
+                const index_t bx = full_x ? ox : hx + ox - 2;
+                const T dx = full_x ? T(0) : slab_distance(c.x, bx * a.wx, a.wx);
+                if (dx * dx + dyz < cull_squared)
+                {
+                    first = ox < first ? ox : first;
+                    last = ox;
+                }
+            }
+            if (last >= first)
+            {
+                const index_t wy = by < 0 ? by + ny : (by >= ny ? by - ny : by);
+                const index_t wz = bz < 0 ? bz + nz : (bz >= nz ? bz - nz : bz);
+                const std::int64_t base =
+                    static_cast<std::int64_t>(nx) * (wy + static_cast<std::int64_t>(ny) * wz);
+                const index_t x0 = full_x ? first : hx + first - 2;
+                const index_t x1 = full_x ? last : hx + last - 2;
+                // a run that wraps around the x faces splits into two contiguous runs
+                index_t run_lo = 0, run_hi = -1;
+                if (x0 < 0)
+                {
+                    run_lo = part == 0 ? x0 + nx : 0;
+                    run_hi = part == 0 ? nx - 1 : x1;
+                }
+                else if (x1 >= nx)
+                {
+                    run_lo = part == 0 ? x0 : 0;
+                    run_hi = part == 0 ? nx - 1 : x1 - nx;
+                }
+                else if (part == 0)
+                {
+                    run_lo = x0;
+                    run_hi = x1;
+                }
+                if (run_hi >= run_lo)
+                {
+                    lo = a.bin_offsets[base + run_lo];
+                    length = a.bin_offsets[base + run_hi + 1] - lo;
+                }
+            }
+        }
+    }
+    const index_t inclusive = wave_inclusive_sum(length);
+    const index_t start = inclusive - length;
+    const index_t total = __shfl(inclusive, kWave - 1, kWave);
+    // near a face (or with a short axis) the images come from the minimum image per candidate
+    const bool wraps = full_x || full_y || full_z || hx < 2 || hx + 2 >= nx || hy < 2 ||
+                       hy + 2 >= ny || hz < 2 || hz + 2 >= nz;
+    index_t count = 0;
+    index_t* row_out = a.index + center * a.stride;
+    for (index_t chunk = 0; chunk < total; chunk += kWave)
+    {
+        const index_t t = chunk + lane;
+        // the range holding t: the last lane whose start is <= t
+        int r = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+        {
+            const index_t s = __shfl(start, r + step, kWave);
+            if (s <= t) r += step;
+        }
+        const index_t slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
+        bool hit = false;
+        if (t < total && slot != center)
+        {
+            const V4 q = pos[slot];
+            T dx, dy, dz;
+            if (wraps)
+            {
+                dx = minimum_image(c.x, q.x, lx);
+                dy = minimum_image(c.y, q.y, ly);
+                dz = minimum_image(c.z, q.z, lz);
+            }
+            else
+            {
+                dx = c.x - q.x;
+                dy = c.y - q.y;
+                dz = c.z - q.z;
+            }
+            const T xx = dx * dx;
+            const T yy = dy * dy;
+            const T zz = dz * dz;
+            hit = (xx + yy) + zz < radius_squared;
+        }
+        const std::uint64_t mask = ballot(hit);
+        if (hit && a.stride > 0)
+        {
+            const index_t position = count + bits_below(mask);
+            if (position < a.stride) row_out[position] = slot;
+        }
+        count += static_cast<index_t>(__popcll(mask));
+    }
+    if (lane == 0)
+    {
+        a.counts[center] = count;
+        if (a.stride > 0 && count > a.stride)
+        {
+            atomicOr(a.status, kStatusFullOverflow);
+            atomicMax(a.status + 1, count);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_commit_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 3LL * a.num_atoms) return;
+    a.reference_positions[i] = a.positions[i];
+    if (i < 3) a.reference_cell[i] = a.cell[i];
+    if (i == 0 && a.rebuild_count) ++a.rebuild_count[0];
+}
+
+// ---- forces ---------------------------------------------------------------------------------
+
+// Blocks are dealt to the 8 XCDs round-robin; renumbering them puts consecutive logical blocks
+// (neighboring rows, sharing neighbor positions) on one XCD and its L2. grid is a multiple of 8.
+__device__ __forceinline__ unsigned xcd_block(unsigned block, unsigned grid)
+{
+    const unsigned per = grid / 8;
+    return (block % 8) * per + block / 8;
+}
+
+inline unsigned xcd_grid(std::int64_t blocks)
+{
+    return static_cast<unsigned>((blocks + 7) / 8 * 8);
+}
+
+template <int G, typename V>
+__device__ __forceinline__ V group_sum(V value)
+{
+#pragma unroll
+    for (int offset = G / 2; offset > 0; offset >>= 1) value += __shfl_xor(value, offset, kWave);
+    return value;
+}
+
+// G lanes per row: each reads G consecutive entries of its row at a time. Wave-uniform choice
+// of images: a wave whose atoms all sit farther than the cutoff (plus a margin) from every face
+// takes x_i - x_j directly, which is the minimum image for every pair within the cutoff; any
+// other image it may pick is longer than L/2 > rc, so the mask drops it exactly as the
+// reference does. Other waves use the reference's displacement (space.py:11-19) per pair.
+template <typename T, class Pot, int G, bool kVirial, bool kKick>
+__global__ void __launch_bounds__(kBlock) force_kernel(StepArgs<T> a, Pot pot, T cutoff,
+                                                       T cutoff_squared, T half_timestep,
+                                                       double* __restrict__ partials,
+                                                       std::int64_t real_blocks)
+{
+    using V4 = typename Vec4<T>::type;
+    __shared__ double shared[16];
+    constexpr int kRowsPerBlock = kBlock / G;
+    const unsigned block = xcd_block(blockIdx.x, gridDim.x);
+    const int lane = lane_id();
+    const int sub = lane % G;
+    const std::int64_t row = static_cast<std::int64_t>(block) * kRowsPerBlock + threadIdx.x / G;
+    const bool live = block < real_blocks && row < a.num_atoms;
+    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
+    V4 c{};
+    index_t count = 0;
+    if (live)
+    {
+        c = pos[row];
+        count = a.counts[row];
+    }
+    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
+    const T low = cutoff + T(1e-3);
+    const bool interior = !live || (c.x > low && c.x < lx - low && c.y > low && c.y < ly - low &&
+                                    c.z > low && c.z < lz - low);
+    const bool fast = ballot(!interior) == 0ull;
+    const index_t width = wave_max(count);
+    const index_t* __restrict__ entries = a.index + (live ? row : 0) * a.stride;
+    T fx = 0, fy = 0, fz = 0, energy = 0;
+    T w[6] = {0, 0, 0, 0, 0, 0};
+    for (index_t k = sub; k < width; k += G)
+    {
+        if (k >= count) continue;
+        const index_t j = entries[k];
+        const V4 q = pos[j];
+        T dx, dy, dz;
+        if (fast)
+        {
+            dx = c.x - q.x;
+            dy = c.y - q.y;
+            dz = c.z - q.z;
+        }
+        else
+        {
+            dx = displacement(c.x, q.x, lx);
+            dy = displacement(c.y, q.y, ly);
+            dz = displacement(c.z, q.z, lz);
+        }
+        const T r2 = exact_squared(dx, dy, dz);
+        if (!(r2 < cutoff_squared)) continue;
+        T u, fs;
+        pot.eval(pot.row(row, j), r2, u, fs);
+        energy += u;
+        fx += fs * dx;
+        fy += fs * dy;
+        fz += fs * dz;
+        if (kVirial)
+        {
+            w[0] += dx * (fs * dx);
+            w[1] += dy * (fs * dy);
+            w[2] += dz * (fs * dz);
+            w[3] += dx * (fs * dy);
+            w[4] += dx * (fs * dz);
+            w[5] += dy * (fs * dz);
+        }
+    }
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    double kinetic = 0.0;
+    if (live && sub == 0)
+    {
+        T* f = a.forces + 3 * row;
+        f[0] = fx;
+        f[1] = fy;
+        f[2] = fz;
+        if (kKick)
+        {
+#pragma clang fp contract(off)
+            T* v = a.velocities + 3 * row;
+            const T inverse_mass = a.inverse_masses[row];
+            const T vx = v[0] + half_timestep * fx * inverse_mass;
+            const T vy = v[1] + half_timestep * fy * inverse_mass;
+            const T vz = v[2] + half_timestep * fz * inverse_mass;
+            v[0] = vx;
+            v[1] = vy;
+            v[2] = vz;
+            kinetic = 0.5 * static_cast<double>(a.masses[row]) *
+                      (static_cast<double>(vx) * vx + static_cast<double>(vy) * vy +
+                       static_cast<double>(vz) * vz);
+        }
+    }
+    store_partials<kVirial>(static_cast<double>(energy), w, shared, partials);
+    const double k_sum = block_sum(kinetic, shared);
+    if (threadIdx.x == 0) partials[8LL * blockIdx.x + 7] = k_sum;
+}
+
+// E, W and K from the partials (pairs counted twice, so E and W halve); then the chain's
+// closing half step, whose velocity factor waits in the thermostat for the next step.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) finish_step_kernel(StepArgs<T> a,
+                                                             const double* __restrict__ partials,
+                                                             int blocks, int kinetic_from_partials)
+{
+    __shared__ double shared[16];
+    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+        for (int k = 0; k < 8; ++k) sums[k] += partials[8LL * b + k];
+    for (int k = 0; k < 8; ++k) sums[k] = block_sum(sums[k], shared);
+    if (threadIdx.x != 0) return;
+    if (a.energy) a.energy[0] = static_cast<T>(0.5 * sums[0]);
+    if (a.virial)
+    {
+        const double xx = 0.5 * sums[1], yy = 0.5 * sums[2], zz = 0.5 * sums[3];
+        const double xy = 0.5 * sums[4], xz = 0.5 * sums[5], yz = 0.5 * sums[6];
+        const double w[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+        for (int k = 0; k < 9; ++k) a.virial[k] = static_cast<T>(w[k]);
+    }
+    if (kinetic_from_partials && a.kinetic) a.kinetic[0] = sums[7];
+    if (a.nvt && kinetic_from_partials)
+    {
+        double* t = a.thermostat;
+        t[kPending] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
+                                      a.degrees, a.kT, a.timestep);
+    }
+}
+
+// Applies the pending closing factor to the velocities (before they are read out).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) settle_kernel(T* velocities, std::int64_t count,
+                                                        const double* thermostat)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) velocities[i] = velocities[i] * static_cast<T>(thermostat[kPending]);
+}
+
+__global__ void settle_reset_kernel(double* thermostat)
+{
+    thermostat[kPending] = 1.0;
+}
+
+// Slot order -> atom order for readout: out[ids[slot]] = in[slot], `width` values per atom.
+template <typename E>
+__global__ void __launch_bounds__(kBlock) to_atom_order_kernel(const E* __restrict__ in,
+                                                               const index_t* __restrict__ ids,
+                                                               std::int64_t num_atoms, int width,
+                                                               E* __restrict__ out)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= num_atoms * width) return;
+    const std::int64_t slot = i / width;
+    out[static_cast<std::int64_t>(ids[slot]) * width + i % width] = in[i];
+}
+
+// ---- host -----------------------------------------------------------------------------------
+
+template <typename T>
+StepArgs<T> step_args(const ndn_md_system& s, const ndn_md_settings* settings,
+                      const ndn_list_state* list, const ndn_md_list& m)
+{
+    StepArgs<T> a{};
+    a.positions = static_cast<T*>(s.positions);
+    a.velocities = static_cast<T*>(s.velocities);
+    a.forces = static_cast<T*>(s.forces);
+    a.images = s.images;
+    a.masses = static_cast<T*>(s.masses);
+    a.inverse_masses = static_cast<T*>(s.inverse_masses);
+    a.cell = static_cast<const T*>(s.cell);
+    a.energy = static_cast<T*>(s.energy);
+    a.virial = static_cast<T*>(s.virial);
+    a.num_atoms = static_cast<index_t>(s.num_atoms);
+    a.thermostat = s.thermostat;
+    a.chain_masses = s.chain_masses;
+    a.kinetic = s.kinetic;
+    if (settings)
+    {
+        a.kT = settings->kT;
+        a.degrees = static_cast<double>(settings->degrees_of_freedom);
+        a.timestep = settings->timestep;
+        a.nvt = settings->thermostat;
+    }
+    if (list)
+    {
+        a.cutoff = static_cast<T>(list->cutoff);
+        a.radius = static_cast<T>(list->radius);
+        a.reference_positions = static_cast<T*>(list->reference_positions);
+        a.reference_cell = static_cast<T*>(list->reference_cell);
+        a.limits = list->limits;
+        a.reuse = static_cast<T*>(list->reuse);
+        a.rebuild = list->rebuild;
+        a.rebuild_count = list->rebuild_count;
+    }
+    a.status = m.status;
+    a.dims_scratch = m.prepare_dims;
+    a.widths_scratch = static_cast<T*>(m.prepare_widths);
+    a.total_scratch = m.prepare_total;
+    a.header = reinterpret_cast<ListHeader*>(m.scan_temp);
+    a.nx = m.dims[0];
+    a.ny = m.dims[1];
+    a.nz = m.dims[2];
+    a.wx = static_cast<T>(m.widths[0]);
+    a.wy = static_cast<T>(m.widths[1]);
+    a.wz = static_cast<T>(m.widths[2]);
+    a.bins = static_cast<std::int64_t>(m.dims[0]) * m.dims[1] * m.dims[2];
+    a.stride = m.stride;
+    a.bin_counts = m.bin_counts;
+    a.bin_offsets = m.bin_offsets;
+    a.bin_of = m.bin_of;
+    a.unordered = m.unordered;
+    a.order = m.order;
+    a.counts = m.counts;
+    a.index = m.index;
+    a.ids = m.ids;
+    a.types = m.types;
+    a.positions4 = static_cast<T*>(m.positions4);
+    a.scratch = static_cast<std::uint8_t*>(m.scratch);
+    a.scan_temp = m.scan_temp;
+    return a;
+}
+
+template <typename T>
+int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream_t stream)
+{
+    const std::int64_t n = a.num_atoms;
+    md_assign_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_assign");
+    const index_t* gate = forced ? nullptr : a.rebuild;
+    const int scanned = exclusive_scan(a.bin_counts, a.bin_offsets, a.bins + 1, a.scan_temp, gate,
+                                       a.status, stream);
+    if (scanned != NDN_OK) return scanned;
+    md_scatter_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_scatter");
+    md_order_kernel<T><<<blocks_for(a.bins), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_order");
+    md_permute_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_permute");
+    md_unpack_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_unpack");
+    const T r = static_cast<T>(radius);
+    const T r2 = r * r;
+    // culling keeps every sub-bin a pair within R could sit in, with room for rounding
+    const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
+    md_fill_kernel<T><<<blocks_for(n, kBlock / kWave), kBlock, 0, stream>>>(a, r2, cull * cull,
+                                                                            forced);
+    NDN_LAUNCHED("md_fill");
+    md_commit_kernel<T><<<blocks_for(3 * n), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_commit");
+    return NDN_OK;
+}
+
+std::int64_t force_blocks(std::int64_t num_atoms, int group)
+{
+    const std::int64_t rows_per_block = kBlock / group;
+    return (num_atoms + rows_per_block - 1) / rows_per_block;
+}
+
+template <typename T, class Pot, int G>
+void launch_force_g(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
+                    T half_timestep, bool virial, bool kick, double* partials, hipStream_t stream)
+{
+    const std::int64_t real = force_blocks(a.num_atoms, G);
+    const unsigned grid = xcd_grid(real);
+    if (virial && kick)
+        force_kernel<T, Pot, G, true, true><<<grid, kBlock, 0, stream>>>(
+            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
+    else if (virial)
+        force_kernel<T, Pot, G, true, false><<<grid, kBlock, 0, stream>>>(
+            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
+    else if (kick)
+        force_kernel<T, Pot, G, false, true><<<grid, kBlock, 0, stream>>>(
+            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
+    else
+        force_kernel<T, Pot, G, false, false><<<grid, kBlock, 0, stream>>>(
+            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
+}
+
+template <typename T, class Pot>
+void launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff, T cutoff_squared,
+                  T half_timestep, bool virial, bool kick, double* partials, hipStream_t stream)
+{
+    switch (group)
+    {
+        case 1:
+            launch_force_g<T, Pot, 1>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
+                                      partials, stream);
+            break;
+        case 4:
+            launch_force_g<T, Pot, 4>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
+                                      partials, stream);
+            break;
+        case 8:
+            launch_force_g<T, Pot, 8>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
+                                      partials, stream);
+            break;
+        case 32:
+            launch_force_g<T, Pot, 32>(a, pot, cutoff, cutoff_squared, half_timestep, virial,
+                                       kick, partials, stream);
+            break;
+        default:
+            launch_force_g<T, Pot, 16>(a, pot, cutoff, cutoff_squared, half_timestep, virial,
+                                       kick, partials, stream);
+    }
+}
+
+int valid_group(int group)
+{
+    return group == 1 || group == 4 || group == 8 || group == 16 || group == 32;
+}
+
+template <typename T>
+int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coefficients, int group,
+                  bool virial, bool kick, double* partials, hipStream_t stream)
+{
+    const T cutoff = static_cast<T>(pot->cutoff);
+    const T cutoff_squared = static_cast<T>(pot->cutoff * pot->cutoff);
+    const T half_timestep = static_cast<T>(0.5 * a.timestep);
+    const int rows = pot->num_types * (pot->num_types + 1) / 2;
+    if (pot->kind == NDN_LJ)
+    {
+        const FastLJ<T> lj{static_cast<const LJRow<T>*>(coefficients), a.types, pot->num_types};
+        launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                        partials, stream);
+    }
+    else
+    {
+        T* padded = static_cast<T*>(coefficients);
+        const Table<T> table{padded, static_cast<const T*>(pot->start),
+                             padded + static_cast<std::int64_t>(rows) * (pot->knots + 2), a.types,
+                             pot->num_types, static_cast<int>(pot->knots)};
+        launch_force<T>(a, table, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                        partials, stream);
+    }
+    NDN_LAUNCHED("md_force");
+    return NDN_OK;
+}
+
+template <typename T>
+int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const ndn_potential* pot,
+                 const ndn_list_state* list, const ndn_md_list& m, hipStream_t stream)
+{
+    StepArgs<T> a = step_args<T>(s, settings, list, m);
+    const T timestep = static_cast<T>(settings->timestep);
+    const T half_timestep = static_cast<T>(0.5 * settings->timestep);
+    prologue_kernel<T><<<1, kBlock, 0, stream>>>(a, *pot, s.coefficients);
+    NDN_LAUNCHED("md_prologue");
+    first_half_kernel<T><<<blocks_for(s.num_atoms), kBlock, 0, stream>>>(a, timestep, half_timestep);
+    NDN_LAUNCHED("md_first_half");
+    const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream);
+    if (rebuilt != NDN_OK) return rebuilt;
+    const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, settings->virial != 0,
+                                        true, s.partials, stream);
+    if (forces != NDN_OK) return forces;
+    const int blocks = static_cast<int>(xcd_grid(force_blocks(s.num_atoms, m.group)));
+    finish_step_kernel<T><<<1, kBlock, 0, stream>>>(a, s.partials, blocks, 1);
+    NDN_LAUNCHED("md_finish");
+    return NDN_OK;
+}
+
+template <typename T>
+int md_evaluate_impl(const ndn_md_system& s, const ndn_potential* pot, const ndn_md_list& m,
+                     int virial, int kick, double timestep, hipStream_t stream)
+{
+    StepArgs<T> a = step_args<T>(s, nullptr, nullptr, m);
+    a.timestep = timestep;
+    PotentialView<T> view;
+    const int prepared = prepare_potential<T>(pot, s.coefficients, stream, &view);
+    if (prepared != NDN_OK) return prepared;
+    const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, virial != 0, kick != 0,
+                                        s.partials, stream);
+    if (forces != NDN_OK) return forces;
+    const int blocks = static_cast<int>(xcd_grid(force_blocks(s.num_atoms, m.group)));
+    finish_step_kernel<T><<<1, kBlock, 0, stream>>>(a, s.partials, blocks, 0);
+    NDN_LAUNCHED("md_evaluate_finish");
+    return NDN_OK;
+}
+
+int check_system(const ndn_md_system* s, const ndn_md_list* m)
+{
+    NDN_REQUIRE(s != nullptr && m != nullptr, "system and list must not be null");
+    NDN_REQUIRE(s->num_atoms >= 1 && s->num_atoms <= INT32_MAX, "N must satisfy 1 <= N < 2**31");
+    NDN_REQUIRE(s->positions && s->velocities && s->forces && s->images && s->masses &&
+                    s->inverse_masses && s->cell && s->energy && s->thermostat && s->kinetic &&
+                    s->partials && s->coefficients,
+                "a system buffer is null");
+    NDN_REQUIRE(m->bin_counts && m->bin_offsets && m->bin_of && m->unordered && m->order &&
+                    m->counts && m->ids && m->positions4 && m->scratch && m->scan_temp &&
+                    m->status && m->prepare_dims && m->prepare_widths && m->prepare_total,
+                "a list buffer is null");
+    NDN_REQUIRE(m->dims[0] >= 1 && m->dims[1] >= 1 && m->dims[2] >= 1, "bad sub-bin dims");
+    const std::int64_t bins = static_cast<std::int64_t>(m->dims[0]) * m->dims[1] * m->dims[2];
+    NDN_REQUIRE(bins < INT32_MAX, "too many sub-bins");
+    NDN_REQUIRE(m->scan_temp_bytes >= scan_bytes(bins + 1) && m->scan_temp_bytes >= 1024,
+                "scan_temp is too small");
+    NDN_REQUIRE(m->stride == 0 || m->index != nullptr, "a list with rows needs its index");
+    NDN_REQUIRE(m->stride >= 0 && m->stride * s->num_atoms <= (1LL << 40), "bad stride");
+    NDN_REQUIRE(valid_group(m->group), "group must be 1, 4, 8, 16 or 32");
+    NDN_REQUIRE(s->partial_slots >= static_cast<std::int64_t>(xcd_grid(force_blocks(s->num_atoms, 1))),
+                "too few partial slots");
+    return NDN_OK;
+}
+
+}  // namespace ndn
+
+using namespace ndn;
+
+extern "C" {
+
+int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
+                     int64_t* partial_slots, int64_t* coefficient_bytes_out)
+{
+    const int checked = check_potential(pot, num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    NDN_REQUIRE(num_atoms >= 1 && partial_slots && coefficient_bytes_out, "bad arguments");
+    *partial_slots = std::max<std::int64_t>(xcd_grid(force_blocks(num_atoms, 1)), 8);
+    *coefficient_bytes_out = coefficient_bytes(pot, dtype == NDN_F32 ? 4 : 8);
+    return NDN_OK;
+}
+
+int64_t ndn_md_scratch_bytes(int dtype, int64_t num_atoms)
+{
+    return num_atoms * (dtype == NDN_F32 ? static_cast<int64_t>(sizeof(Packed<float>))
+                                         : static_cast<int64_t>(sizeof(Packed<double>)));
+}
+
+int ndn_md_rebuild(int dtype, const ndn_md_system* system, const ndn_list_state* list,
+                   const ndn_md_list* md_list, int forced, ndn_stream_t stream)
+{
+    const int checked = check_system(system, md_list);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(list != nullptr, "the list state must not be null");
+    NDN_REQUIRE(forced || list->rebuild != nullptr, "an unforced rebuild needs the flag");
+    if (dtype == NDN_F32)
+    {
+        const StepArgs<float> a = step_args<float>(*system, nullptr, list, *md_list);
+        return launch_md_rebuild<float>(a, list->radius, forced, stream);
+    }
+    NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    const StepArgs<double> a = step_args<double>(*system, nullptr, list, *md_list);
+    return launch_md_rebuild<double>(a, list->radius, forced, stream);
+}
+
+int ndn_md_step(int dtype, const ndn_md_system* system, const ndn_md_settings* settings,
+                const ndn_potential* pot, const ndn_list_state* list, const ndn_md_list* md_list,
+                ndn_stream_t stream)
+{
+    int checked = check_system(system, md_list);
+    if (checked != NDN_OK) return checked;
+    checked = check_potential(pot, system->num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(settings && list, "settings and list must not be null");
+    NDN_REQUIRE(settings->timestep > 0, "the timestep must be positive");
+    NDN_REQUIRE(!settings->thermostat || system->chain_masses, "NVT needs chain masses");
+    NDN_REQUIRE(list->cutoff == pot->cutoff, "the list's cutoff must be the potential's");
+    NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
+    NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
+                "several atom types need the slot-ordered types");
+    if (dtype == NDN_F32)
+        return md_step_impl<float>(*system, settings, pot, list, *md_list, stream);
+    NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    return md_step_impl<double>(*system, settings, pot, list, *md_list, stream);
+}
+
+int ndn_md_evaluate(int dtype, const ndn_md_system* system, const ndn_potential* pot,
+                    const ndn_md_list* md_list, int virial, int kick, double timestep,
+                    ndn_stream_t stream)
+{
+    int checked = check_system(system, md_list);
+    if (checked != NDN_OK) return checked;
+    checked = check_potential(pot, system->num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
+    NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
+                "several atom types need the slot-ordered types");
+    if (dtype == NDN_F32)
+        return md_evaluate_impl<float>(*system, pot, *md_list, virial, kick, timestep, stream);
+    NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    return md_evaluate_impl<double>(*system, pot, *md_list, virial, kick, timestep, stream);
+}
+
+int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* pot,
+                 const ndn_md_list* md_list, int virial, int kick, double timestep,
+                 ndn_stream_t stream)
+{
+    int checked = check_system(system, md_list);
+    if (checked != NDN_OK) return checked;
+    checked = check_potential(pot, system->num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
+    if (dtype == NDN_F32)
+    {
+        StepArgs<float> a = step_args<float>(*system, nullptr, nullptr, *md_list);
+        a.timestep = timestep;
+        return launch_forces<float>(a, pot, system->coefficients, md_list->group, virial != 0,
+                                    kick != 0, system->partials, stream);
+    }
+    NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    StepArgs<double> a = step_args<double>(*system, nullptr, nullptr, *md_list);
+    a.timestep = timestep;
+    return launch_forces<double>(a, pot, system->coefficients, md_list->group, virial != 0,
+                                 kick != 0, system->partials, stream);
+}
+
+int ndn_md_settle(int dtype, const ndn_md_system* system, ndn_stream_t stream)
+{
+    NDN_REQUIRE(system && system->velocities && system->thermostat, "bad arguments");
+    const std::int64_t count = 3 * system->num_atoms;
+    if (dtype == NDN_F32)
+        settle_kernel<float><<<blocks_for(count), kBlock, 0, stream>>>(
+            static_cast<float*>(system->velocities), count, system->thermostat);
+    else
+        settle_kernel<double><<<blocks_for(count), kBlock, 0, stream>>>(
+            static_cast<double*>(system->velocities), count, system->thermostat);
+    NDN_LAUNCHED("md_settle");
+    settle_reset_kernel<<<1, 1, 0, stream>>>(system->thermostat);
+    NDN_LAUNCHED("md_settle_reset");
+    return NDN_OK;
+}
+
+int ndn_md_to_atom_order(int element_bytes, const void* in, const int32_t* ids, int64_t num_atoms,
+                         int width, void* out, ndn_stream_t stream)
+{
+    NDN_REQUIRE(in && ids && out && num_atoms >= 1 && width >= 1, "bad arguments");
+    const std::int64_t count = num_atoms * width;
+    if (element_bytes == 4)
+        to_atom_order_kernel<std::int32_t><<<blocks_for(count), kBlock, 0, stream>>>(
+            static_cast<const std::int32_t*>(in), ids, num_atoms, width,
+            static_cast<std::int32_t*>(out));
+    else
+    {
+        NDN_REQUIRE(element_bytes == 8, "element_bytes must be 4 or 8");
+        to_atom_order_kernel<std::int64_t><<<blocks_for(count), kBlock, 0, stream>>>(
+            static_cast<const std::int64_t*>(in), ids, num_atoms, width,
+            static_cast<std::int64_t*>(out));
+    }
+    NDN_LAUNCHED("md_to_atom_order");
+    return NDN_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "neighbors.hpp"
 #include "scan.hpp"
 
 namespace ndn {
@@ -34,31 +35,7 @@ const char* last_error()
     return g_last_error.c_str();
 }
 
-template <typename T>
-struct Epsilon;
-template <>
-struct Epsilon<float>
-{
-    static constexpr float value = std::numeric_limits<float>::epsilon();
-};
-template <>
-struct Epsilon<double>
-{
-    static constexpr double value = std::numeric_limits<double>::epsilon();
-};
-
 // ---- geometry -------------------------------------------------------------------------------
-
-// center - other with the image chosen per candidate: the coordinate near the far face moves by
-// L first (exact), so the difference rounds once.
-template <typename T>
-__device__ __forceinline__ T minimum_image(T center, T other, T length)
-{
-    const T raw = center - other;
-    return raw > T(0.5) * length ? (center - length) - other
-         : raw < T(-0.5) * length ? center - (other - length)
-         : raw;
-}
 
 __device__ __forceinline__ index_t wrap_bin(index_t home, int step, index_t count)
 {
@@ -95,51 +72,8 @@ __global__ void prepare_update_kernel(const T* __restrict__ cell,
                                       T* reuse, index_t* rebuild, index_t* status,
                                       const index_t* pair_total, ListHeader* header)
 {
-    const T epsilon = Epsilon<T>::value;
-    header->old_total = pair_total[0];
-    header->max_moved_bits = 0ull;
-    rebuild[0] = 0;
-    if (status[0] != 0) return;
-    T smallest_scale = cell[0] / reference_cell[0];
-    T longest = cell[0];
-    for (int axis = 0; axis < 3; ++axis)
-    {
-        const T length = cell[axis];
-        if (!(isfinite(length) && length > 2 * cutoff))
-        {
-            status[0] |= kStatusInvalidCell;
-            return;
-        }
-        const T scale = length / reference_cell[axis];
-        reuse[1 + axis] = scale;
-        smallest_scale = scale < smallest_scale ? scale : smallest_scale;
-        longest = length > longest ? length : longest;
-        const T fit = floor(length / (radius + 4 * epsilon * length));
-        const index_t count =
-            fit < 1 ? 1 : (fit > limits[axis] ? limits[axis] : static_cast<index_t>(fit));
-        dims[axis] = count;
-        widths[axis] = length / count;
-    }
-    const T allowance = radius * smallest_scale - cutoff - 32 * epsilon * longest;
-    if (allowance <= 0)
-    {
-        rebuild[0] = 1;
-        return;
-    }
-    reuse[0] = allowance * allowance / 4;
-}
-
-template <typename T>
-__device__ __forceinline__ unsigned long long order_bits(T value);
-template <>
-__device__ __forceinline__ unsigned long long order_bits<float>(float value)
-{
-    return static_cast<unsigned long long>(__float_as_uint(value));
-}
-template <>
-__device__ __forceinline__ unsigned long long order_bits<double>(double value)
-{
-    return static_cast<unsigned long long>(__double_as_longlong(value));
+    prepare_update(cell, reference_cell, cutoff, radius, limits, dims, widths, reuse, rebuild,
+                   status, pair_total, header);
 }
 
 // One atom per lane: flags a non-finite coordinate, and reduces the squared displacement since
@@ -151,31 +85,31 @@ __global__ void __launch_bounds__(kBlock) check_reuse_kernel(
     const T* __restrict__ cell, const T* __restrict__ reuse, index_t num_atoms, index_t* rebuild,
     index_t* status, ListHeader* header)
 {
-    if (status[0] != 0) return;
+    __shared__ T block_max[kBlock / kWave];
+    // every thread reaches the barrier below, whatever status a wave reads
+    const bool active = status[0] == 0;
     const std::int64_t atom = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     bool finite = true;
     T moved = 0;
-    if (atom < num_atoms)
+    if (active && atom < num_atoms)
     {
         const T x = positions[3 * atom], y = positions[3 * atom + 1], z = positions[3 * atom + 2];
         finite = isfinite(x) && isfinite(y) && isfinite(z);
-        if (finite)
-        {
-            const T* x0 = reference_positions + 3 * atom;
-            const T dx = minimum_image(x, reuse[1] * x0[0], cell[0]);
-            const T dy = minimum_image(y, reuse[2] * x0[1], cell[1]);
-            const T dz = minimum_image(z, reuse[3] * x0[2], cell[2]);
-            moved = T(0) + dx * dx;
-            moved = moved + dy * dy;
-            moved = moved + dz * dz;
-        }
+        if (finite) moved = moved_squared(x, y, z, reference_positions + 3 * atom, reuse, cell);
     }
     if (!finite) atomicOr(status, kStatusNonFinite);
     const T wave_moved = wave_max(finite ? moved : T(0));
     if (lane_id() == 0)
     {
-        if (wave_moved > T(0)) atomicMax(&header->max_moved_bits, order_bits(wave_moved));
-        if (wave_moved >= reuse[0] && rebuild[0] == 0) rebuild[0] = 1;
+        block_max[threadIdx.x / kWave] = wave_moved;
+        if (active && wave_moved >= reuse[0] && rebuild[0] == 0) rebuild[0] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        T m = block_max[0];
+        for (int w = 1; w < kBlock / kWave; ++w) m = block_max[w] > m ? block_max[w] : m;
+        if (m > T(0)) atomicMax(moved_slot(header, blockIdx.x), order_bits(m));
     }
 }
 
@@ -752,6 +686,33 @@ int launch_fill(const T* cell, double radius, const ndn_neighbor_buffers& b,
     return NDN_OK;
 }
 
+template <typename T>
+int launch_rebuild(const T* positions, const T* cell, double radius, const ndn_neighbor_buffers& b,
+                   T* reference_positions, T* reference_cell, index_t* rebuild_count,
+                   const index_t* rebuild, hipStream_t stream)
+{
+    const index_t n = static_cast<index_t>(b.num_atoms);
+    if (b.bits_words == 0)
+    {
+        const int grid = launch_grid<T>(positions, b, rebuild, stream);
+        if (grid != NDN_OK) return grid;
+    }
+    int step = launch_count<T>(positions, cell, radius, b, rebuild, stream);
+    if (step != NDN_OK) return step;
+    step = launch_fill<T>(cell, radius, b, rebuild, false, stream);
+    if (step != NDN_OK) return step;
+    commit_rebuild_kernel<T><<<blocks_for(3LL * n), kBlock, 0, stream>>>(
+        positions, cell, n, reference_positions, reference_cell, rebuild_count, rebuild, b.status);
+    NDN_LAUNCHED("commit_rebuild");
+    return NDN_OK;
+}
+
+template int launch_rebuild<float>(const float*, const float*, double, const ndn_neighbor_buffers&,
+                                   float*, float*, index_t*, const index_t*, hipStream_t);
+template int launch_rebuild<double>(const double*, const double*, double,
+                                    const ndn_neighbor_buffers&, double*, double*, index_t*,
+                                    const index_t*, hipStream_t);
+
 int check_buffers(const ndn_neighbor_buffers* b)
 {
     NDN_REQUIRE(b != nullptr, "buffers must not be null");
@@ -816,20 +777,8 @@ int update_impl(const void* positions, const void* cell, double cutoff, double r
         x, static_cast<const T*>(reference_positions), L, static_cast<const T*>(reuse), n, rebuild,
         b.status, header);
     NDN_LAUNCHED("check_reuse");
-    if (b.bits_words == 0)
-    {
-        const int grid = launch_grid<T>(x, b, rebuild, stream);
-        if (grid != NDN_OK) return grid;
-    }
-    int step = launch_count<T>(x, L, radius, b, rebuild, stream);
-    if (step != NDN_OK) return step;
-    step = launch_fill<T>(L, radius, b, rebuild, false, stream);
-    if (step != NDN_OK) return step;
-    commit_rebuild_kernel<T><<<blocks_for(3LL * n), kBlock, 0, stream>>>(
-        x, L, n, static_cast<T*>(reference_positions), static_cast<T*>(reference_cell),
-        rebuild_count, rebuild, b.status);
-    NDN_LAUNCHED("commit_rebuild");
-    return NDN_OK;
+    return launch_rebuild<T>(x, L, radius, b, static_cast<T*>(reference_positions),
+                             static_cast<T*>(reference_cell), rebuild_count, rebuild, stream);
 }
 
 }  // namespace ndn

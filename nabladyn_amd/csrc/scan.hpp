@@ -11,15 +11,22 @@ namespace ndn {
 constexpr int kScanPerThread = 4;
 constexpr int kScanTile = kBlock * kScanPerThread;  // 1024 values per block
 
-// Bytes of the list header at the start of scan_temp; the tile sums follow it.
-constexpr std::int64_t kHeaderBytes = 256;
+// Bytes of the list header at the start of scan_temp; the tile sums follow it. Bytes 64-575 hold
+// 64 slots of per-block maxima of the squared displacement at the last reuse check (the bits of
+// a nonnegative T, widened to 64 bits), so no slot takes many atomics.
+constexpr std::int64_t kHeaderBytes = 1024;
+constexpr int kMovedSlots = 64;
 
 struct ListHeader
 {
-    index_t old_total;                 // pair_total before this build: columns past it hold padding
-    index_t pad0;
-    unsigned long long max_moved_bits; // largest squared displacement at the last reuse check
+    index_t old_total;  // pair_total before this build: columns past it hold padding
 };
+
+__device__ __forceinline__ unsigned long long* moved_slot(ListHeader* header, unsigned block)
+{
+    return reinterpret_cast<unsigned long long*>(reinterpret_cast<std::uint8_t*>(header) + 64) +
+           (block % kMovedSlots);
+}
 
 inline std::int64_t scan_tiles(std::int64_t items)
 {
@@ -31,7 +38,7 @@ inline std::int64_t scan_bytes(std::int64_t items)
     return kHeaderBytes + 4 * (scan_tiles(items) + 1);
 }
 
-__global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
+static __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
     const index_t* __restrict__ input, std::int64_t count, index_t* __restrict__ tile_sums,
     const index_t* rebuild, const index_t* status)
 {
@@ -49,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = sum;
 }
 
-__global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
+static __global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
     index_t* __restrict__ tile_sums, std::int64_t tiles, const index_t* rebuild,
     const index_t* status)
 {
@@ -82,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
 }
 
 // out may alias in: each block reads its whole tile before writing it.
-__global__ void __launch_bounds__(kBlock) scan_apply_kernel(
+static __global__ void __launch_bounds__(kBlock) scan_apply_kernel(
     const index_t* input, std::int64_t count, const index_t* __restrict__ tile_prefix,
     index_t* output, const index_t* rebuild, const index_t* status)
 {

@@ -1,0 +1,559 @@
+"""The simulation (``md/_simulation.py``): a resident frame, its neighbor list, and one HIP graph
+holding a whole MD step, replayed by ``run``.
+
+Where the reference compiles the ensemble's halves with Inductor and captures them around the
+neighbor operator, here the step is the library's ``ndn_md_step`` — prologue, first half with
+the reuse check, gated rebuild, fused force and second kick, finish — captured once by
+``torch.cuda.CUDAGraph`` (a hipGraph on ROCm). The neighbor status is read every 1,000 steps,
+as in the reference.
+"""
+
+import math
+import sys
+from numbers import Integral, Real
+
+import torch
+from torch import Tensor
+
+from .. import _native, space
+from ..potential import Potential
+from ..units import BOLTZMANN
+from ._state import ChainState, State
+
+_STATUS_INTERVAL = 1000
+FIELDS = ("potential_energy", "kinetic_energy", "total_energy", "temperature", "pressure")
+
+
+class MDSystem(_native.ctypes.Structure):
+    """``ndn_md_system``."""
+
+    _fields_ = [
+        ("num_atoms", _native.ctypes.c_int64), ("positions", _native.ctypes.c_void_p),
+        ("velocities", _native.ctypes.c_void_p), ("forces", _native.ctypes.c_void_p),
+        ("images", _native.ctypes.c_void_p), ("masses", _native.ctypes.c_void_p),
+        ("inverse_masses", _native.ctypes.c_void_p), ("cell", _native.ctypes.c_void_p),
+        ("energy", _native.ctypes.c_void_p), ("virial", _native.ctypes.c_void_p),
+        ("thermostat", _native.ctypes.c_void_p), ("chain_masses", _native.ctypes.c_void_p),
+        ("kinetic", _native.ctypes.c_void_p), ("partials", _native.ctypes.c_void_p),
+        ("partial_slots", _native.ctypes.c_int64), ("coefficients", _native.ctypes.c_void_p),
+        ("coefficient_bytes", _native.ctypes.c_int64),
+    ]
+
+
+class MDSettings(_native.ctypes.Structure):
+    """``ndn_md_settings``."""
+
+    _fields_ = [
+        ("timestep", _native.ctypes.c_double), ("kT", _native.ctypes.c_double),
+        ("degrees_of_freedom", _native.ctypes.c_int64), ("thermostat", _native.ctypes.c_int),
+        ("virial", _native.ctypes.c_int),
+    ]
+
+
+class ListState(_native.ctypes.Structure):
+    """``ndn_list_state``."""
+
+    _fields_ = [
+        ("cutoff", _native.ctypes.c_double), ("radius", _native.ctypes.c_double),
+        ("reference_positions", _native.ctypes.c_void_p), ("reference_cell", _native.ctypes.c_void_p),
+        ("limits", _native.ctypes.c_void_p), ("reuse", _native.ctypes.c_void_p),
+        ("rebuild", _native.ctypes.c_void_p), ("rebuild_count", _native.ctypes.c_void_p),
+    ]
+
+
+class MDList(_native.ctypes.Structure):
+    """``ndn_md_list``."""
+
+    _fields_ = [
+        ("dims", _native.ctypes.c_int32 * 3), ("group", _native.ctypes.c_int32),
+        ("widths", _native.ctypes.c_double * 3), ("stride", _native.ctypes.c_int64),
+        ("bin_counts", _native.ctypes.c_void_p), ("bin_offsets", _native.ctypes.c_void_p),
+        ("bin_of", _native.ctypes.c_void_p), ("unordered", _native.ctypes.c_void_p),
+        ("order", _native.ctypes.c_void_p), ("counts", _native.ctypes.c_void_p),
+        ("index", _native.ctypes.c_void_p), ("ids", _native.ctypes.c_void_p),
+        ("types", _native.ctypes.c_void_p), ("positions4", _native.ctypes.c_void_p),
+        ("scratch", _native.ctypes.c_void_p), ("scan_temp", _native.ctypes.c_void_p),
+        ("scan_temp_bytes", _native.ctypes.c_int64), ("status", _native.ctypes.c_void_p),
+        ("prepare_dims", _native.ctypes.c_void_p), ("prepare_widths", _native.ctypes.c_void_p),
+        ("prepare_total", _native.ctypes.c_void_p),
+    ]
+
+
+_GROUPS = (1, 4, 8, 16, 32)
+_INVALID_CELL, _NON_FINITE, _OVERFLOW, _FULL_OVERFLOW = 1, 2, 8, 16
+
+
+def sub_bin_dims(lengths: list[float], radius: float, dtype: torch.dtype) -> list[int]:
+    """Sub-bins per axis, each at least radius / 2 wide with a margin for the rounding of binning
+    (so every pair within the radius lies within two sub-bins along each axis)."""
+    eps = torch.finfo(dtype).eps
+    half = 0.5 * radius * (1.0 + 1e-6)
+    return [max(1, math.floor(length / (half + 4 * eps * length))) for length in lengths]
+
+
+class _MDNeighbors:
+    """The MD path's neighbor state: the reuse rule's tensors and the full list in slot order."""
+
+    def __init__(self, cutoff, skin, multiplier, positions, cell, types, group_hint=None):
+        from ..neighbors._neighbor_list import grid_dims
+
+        self.cutoff, self.skin, self.multiplier = float(cutoff), float(skin), float(multiplier)
+        n, device, dtype = positions.shape[0], positions.device, positions.dtype
+        lengths = [float(v) for v in cell.tolist()]
+        ints = {"dtype": torch.int32, "device": device}
+        self.limits = torch.tensor(grid_dims(lengths, self.radius, dtype), **ints)
+        dims = sub_bin_dims(lengths, self.radius, dtype)
+        bins = math.prod(dims)
+        if bins >= 2**31 - 1:
+            raise ValueError("the grid must have fewer than 2**31 - 1 bins")
+        self.reference_positions = positions.clone()
+        self.reference_cell = torch.full((3,), math.inf, dtype=dtype, device=device)
+        self.parameters = torch.zeros(4, dtype=dtype, device=device)
+        self.rebuild = torch.zeros(1, **ints)
+        self.rebuild_count = torch.zeros(1, **ints)
+        scan = _native.ctypes.c_int64()
+        _native.call("scan_temp_bytes", _native.ctypes.c_int64(bins + 1), _native.ctypes.byref(scan))
+        scratch = _native.lib().ndn_md_scratch_bytes
+        scratch.restype = _native.ctypes.c_int64
+        self.buffers = {
+            "bin_counts": torch.zeros(bins + 1, **ints),
+            "bin_offsets": torch.zeros(bins + 1, **ints),
+            "bin_of": torch.zeros(n, **ints),
+            "unordered": torch.zeros(n, **ints),
+            "order": torch.zeros(n, **ints),
+            "counts": torch.zeros(n, **ints),
+            "index": torch.zeros(1, **ints),
+            "ids": torch.arange(n, **ints),
+            "types": types.clone() if types is not None else None,
+            "positions4": torch.zeros((n, 4), dtype=dtype, device=device),
+            "scratch": torch.zeros(int(scratch(_native.dtype_tag(dtype), _native.ctypes.c_int64(n))),
+                                   dtype=torch.uint8, device=device),
+            "scan_temp": torch.zeros(max(scan.value, 4096), dtype=torch.uint8, device=device),
+            "status": torch.zeros(2, **ints),
+            "prepare_dims": torch.zeros(3, **ints),
+            "prepare_widths": torch.zeros(3, dtype=dtype, device=device),
+            "prepare_total": torch.zeros(1, **ints),
+        }
+        b = self.buffers
+        self.native = MDList(
+            dims=(_native.ctypes.c_int32 * 3)(*dims),
+            group=16 if group_hint is None else group_hint,
+            widths=(_native.ctypes.c_double * 3)(*[L / d for L, d in zip(lengths, dims)]),
+            stride=0,
+            **{k: _native.ptr(v) for k, v in b.items() if k not in ("scan_temp",)},
+            scan_temp=_native.ptr(b["scan_temp"]),
+            scan_temp_bytes=b["scan_temp"].numel(),
+        )
+        self.state = ListState(
+            cutoff=self.cutoff, radius=self.radius,
+            reference_positions=_native.ptr(self.reference_positions),
+            reference_cell=_native.ptr(self.reference_cell), limits=_native.ptr(self.limits),
+            reuse=_native.ptr(self.parameters), rebuild=_native.ptr(self.rebuild),
+            rebuild_count=_native.ptr(self.rebuild_count),
+        )
+        self.dims = dims
+
+    @property
+    def radius(self) -> float:
+        return self.cutoff + self.skin
+
+    def size_rows(self, max_count: int) -> None:
+        """Allocate the rows for `max_count` neighbors times the capacity multiplier."""
+        stride = max(8, math.ceil(self.multiplier * max_count))
+        stride = (stride + 7) // 8 * 8
+        n = self.buffers["counts"].numel()
+        self.buffers["index"] = torch.zeros(n * stride, dtype=torch.int32,
+                                            device=self.buffers["counts"].device)
+        self.native.index = _native.ptr(self.buffers["index"])
+        self.native.stride = stride
+
+    @property
+    def stride(self) -> int:
+        return int(self.native.stride)
+
+    @property
+    def _rebuild_count(self) -> int:
+        return int(self.rebuild_count.item())
+
+    @property
+    def num_pairs(self) -> int:
+        """Pairs within the list radius at the last build (each counted once)."""
+        return int(self.buffers["counts"].sum(dtype=torch.int64).item()) // 2
+
+    def _max_moved(self) -> float:
+        """The largest squared displacement measured at the last reuse check, in nm^2."""
+        raw = self.buffers["scan_temp"][64:64 + 8 * 64].view(torch.int64).max().item()
+        import struct
+
+        if self.reference_positions.dtype == torch.float32:
+            return struct.unpack("<f", struct.pack("<I", raw & 0xFFFFFFFF))[0]
+        return struct.unpack("<d", struct.pack("<q", raw))[0]
+
+    def _check_status(self) -> None:
+        flags, needed = self.buffers["status"].tolist()
+        if flags:
+            causes = []
+            if flags & _INVALID_CELL:
+                causes.append("every cell length must be finite and above twice the cutoff")
+            if flags & _NON_FINITE:
+                causes.append("positions must be finite")
+            if flags & _FULL_OVERFLOW:
+                causes.append(
+                    f"a neighbor row needs {needed} entries but holds {self.stride}; "
+                    "raise capacity_multiplier",
+                )
+            raise ValueError("; ".join(causes))
+
+    def atom_pairs(self):
+        """The full list as [P, 2] atom-id pairs (both orientations), on the host; for tests."""
+        b = self.buffers
+        counts = b["counts"].cpu()
+        stride = self.stride
+        index = b["index"].view(-1, stride).cpu()
+        ids = b["ids"].cpu()
+        rows = torch.repeat_interleave(torch.arange(counts.numel()), counts)
+        columns = torch.cat([torch.arange(int(c)) for c in counts]) if counts.numel() else counts
+        neighbors = index[rows, columns].long()
+        return torch.stack([ids[rows].long(), ids[neighbors]], dim=1)
+
+
+def _positive(name: str, value) -> float:
+    if isinstance(value, bool) or not isinstance(value, Real) or not math.isfinite(value) or value <= 0:
+        raise ValueError(f"{name} must be a finite positive number")
+    return float(value)
+
+
+def _count(name: str, value) -> int:
+    if isinstance(value, bool) or not isinstance(value, Integral) or value < 0:
+        raise ValueError(f"{name} must be a nonnegative integer")
+    return int(value)
+
+
+def _check_frame(model, positions, velocities, masses, cell) -> None:
+    """``md/_simulation.py:66-88``."""
+    num_atoms, dtype, device = model.num_atoms, model.dtype, model.device
+    for name, value, shape in (
+        ("positions", positions, (num_atoms, 3)), ("velocities", velocities, (num_atoms, 3)),
+        ("masses", masses, (num_atoms,)), ("cell", cell, (3,)),
+    ):
+        if not isinstance(value, Tensor) or value.shape != shape or value.dtype != dtype or value.device != device:
+            raise ValueError(
+                f"{name} must be a {list(shape)} tensor of {dtype} on {device} to match the model",
+            )
+
+
+def _check_chain(chain, device) -> None:
+    for field, value in zip(("positions", "momenta"), chain):
+        if not isinstance(value, Tensor) or value.shape != (3,) or value.dtype != torch.float64 or value.device != device:
+            raise ValueError(f"thermostat.{field} must be a [3] float64 tensor on {device}")
+
+
+class Simulation:
+    """Molecular dynamics of a potential on the GPU; build it with ``nve`` or ``nvt``.
+
+    It owns its frame, a neighbor list at the model's cutoff (with a full list for the fused
+    force kernel) and a HIP graph of one step, which ``run`` replays.
+    """
+
+    def __init__(self, model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor,
+                 cell: Tensor, *, timestep: Real, skin: Real, capacity_multiplier: Real,
+                 images: Tensor | None, step: Integral, temperature: Real | None = None,
+                 thermostat_timescale: Real | None = None, thermostat: ChainState | None = None,
+                 virial_every_step: bool = False):
+        if not isinstance(model, Potential):
+            raise TypeError("model must be a Potential")
+        num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
+        _check_frame(model, positions, velocities, masses, cell)
+        if images is None:
+            images = torch.zeros((num_atoms, 3), dtype=torch.int64, device=device)
+        elif not isinstance(images, Tensor) or images.shape != (num_atoms, 3) or images.dtype != torch.int64 or images.device != device:
+            raise ValueError(f"images must be a [{num_atoms}, 3] int64 tensor on {device}")
+        timestep = _positive("timestep", timestep)
+        self._step = _count("step", step)
+        velocities, masses = velocities.detach(), masses.detach().clone()
+        if not bool(torch.isfinite(velocities).all()):
+            raise ValueError("velocities must be finite")
+        if not bool((torch.isfinite(masses) & (masses > 0)).all()):
+            raise ValueError("masses must be finite and positive")
+        self._nvt = temperature is not None
+        if self._nvt:
+            _positive("temperature", temperature)
+            _positive("thermostat_timescale", thermostat_timescale)
+            if num_atoms < 2:
+                raise ValueError("NVT needs at least 2 atoms, for 3N - 3 degrees of freedom")
+        cell = cell.detach().clone().contiguous()
+        wrapped, crossings = space.wrap(positions.detach().contiguous(), cell)
+        if not bool(torch.isfinite(wrapped).all()):
+            raise ValueError("positions must be finite")
+        lengths = [float(v) for v in cell.tolist()]
+        if not all(math.isfinite(v) and v > 2 * model.cutoff for v in lengths):
+            raise ValueError("every cell length must be finite and above twice the cutoff")
+        column = masses[:, None]
+        velocities = velocities - (column * velocities).sum(dim=0) / masses.sum()
+        self._model, self._timestep = model, timestep
+        self._dtype, self._device = dtype, device
+        self._num_atoms = num_atoms
+        self._degrees_of_freedom = 3 * num_atoms - 3
+        self._virial_every_step = bool(virial_every_step)
+        wide = {"dtype": torch.float64, "device": device}
+        # resident tensors in slot order: their addresses are fixed for the graph
+        self.positions = wrapped.contiguous()
+        self.velocities = velocities.contiguous().clone()
+        self.forces = torch.zeros_like(self.positions)
+        self.images = (images + crossings).contiguous()
+        self.cell = cell
+        self.slot_masses = masses.contiguous().clone()
+        self.inverse_masses = (1.0 / masses).contiguous()
+        self._masses = masses
+        self.energy = torch.zeros((), dtype=dtype, device=device)
+        self.virial = torch.zeros((3, 3), dtype=dtype, device=device)
+        self.thermostat = torch.zeros(8, **wide)
+        self.thermostat[6] = 1.0
+        self.thermostat[7] = 1.0
+        self.kinetic = torch.zeros(1, **wide)
+        self.settings = {"ensemble": "NVT" if self._nvt else "NVE"}
+        if self._nvt:
+            self._kT = BOLTZMANN * float(temperature)
+            tau = float(thermostat_timescale)
+            base = self._kT * tau * tau
+            self.chain_masses = torch.tensor([self._degrees_of_freedom * base, base, base], **wide)
+            if thermostat is not None:
+                _check_chain(thermostat, device)
+                self.thermostat[0:3] = thermostat[0]
+                self.thermostat[3:6] = thermostat[1]
+            self.settings.update(temperature=float(temperature), thermostat_timescale=tau)
+        else:
+            self._kT = 0.0
+            self.chain_masses = torch.zeros(3, **wide)
+        pot, self._keep = model._native()
+        self._pot = pot
+        types = model.atom_types if getattr(model, "_num_types", 1) > 1 else None
+        self._neighbors = _MDNeighbors(model.cutoff, skin, capacity_multiplier, self.positions,
+                                       cell, types)
+        if types is not None:
+            pot.atom_types = _native.ptr(self._neighbors.buffers["types"])
+        slots, coef = _native.ctypes.c_int64(), _native.ctypes.c_int64()
+        _native.call("md_workspace", _native.dtype_tag(dtype), _native.ctypes.byref(pot),
+                     _native.ctypes.c_int64(num_atoms), _native.ctypes.byref(slots),
+                     _native.ctypes.byref(coef))
+        self.partials = torch.zeros(slots.value * 8, **wide)
+        self.coefficients = torch.zeros(max(coef.value, 256), dtype=torch.uint8, device=device)
+        self._system = MDSystem(
+            num_atoms=num_atoms, positions=_native.ptr(self.positions),
+            velocities=_native.ptr(self.velocities), forces=_native.ptr(self.forces),
+            images=_native.ptr(self.images), masses=_native.ptr(self.slot_masses),
+            inverse_masses=_native.ptr(self.inverse_masses), cell=_native.ptr(self.cell),
+            energy=_native.ptr(self.energy.view(1)), virial=_native.ptr(self.virial),
+            thermostat=_native.ptr(self.thermostat), chain_masses=_native.ptr(self.chain_masses),
+            kinetic=_native.ptr(self.kinetic), partials=_native.ptr(self.partials),
+            partial_slots=slots.value, coefficients=_native.ptr(self.coefficients),
+            coefficient_bytes=self.coefficients.numel(),
+        )
+        self._settings = MDSettings(
+            timestep=timestep, kT=self._kT, degrees_of_freedom=self._degrees_of_freedom,
+            thermostat=1 if self._nvt else 0, virial=1 if self._virial_every_step else 0,
+        )
+        # the first build: sort and count, size the rows, then fill them
+        self._rebuild(forced=True)
+        self._neighbors._check_status()
+        self._neighbors.size_rows(int(self._neighbors.buffers["counts"].max()))
+        self._rebuild(forced=True)
+        self._neighbors._check_status()
+        self._neighbors.rebuild_count.fill_(1)
+        self.group = self._tune_group()
+        self._evaluate(virial=True)
+        self.kinetic.copy_(0.5 * (self.slot_masses.double()[:, None]
+                                  * self.velocities.double().square()).sum())
+        self._failure: str | None = None
+        self._graph = self._capture()
+
+    # -- the step ---------------------------------------------------------------------------
+
+    def _rebuild(self, forced: bool) -> None:
+        _native.call("md_rebuild", _native.dtype_tag(self._dtype), _native.ctypes.byref(self._system),
+                     _native.ctypes.byref(self._neighbors.state),
+                     _native.ctypes.byref(self._neighbors.native), 1 if forced else 0,
+                     _native.stream(self._device))
+
+    def _launch_step(self) -> None:
+        _native.call(
+            "md_step", _native.dtype_tag(self._dtype), _native.ctypes.byref(self._system),
+            _native.ctypes.byref(self._settings), _native.ctypes.byref(self._pot),
+            _native.ctypes.byref(self._neighbors.state),
+            _native.ctypes.byref(self._neighbors.native), _native.stream(self._device),
+        )
+
+    def _evaluate(self, virial: bool, kick: bool = False) -> None:
+        _native.call(
+            "md_evaluate", _native.dtype_tag(self._dtype), _native.ctypes.byref(self._system),
+            _native.ctypes.byref(self._pot), _native.ctypes.byref(self._neighbors.native),
+            1 if virial else 0, 1 if kick else 0, _native.ctypes.c_double(self._timestep),
+            _native.stream(self._device),
+        )
+
+    def _tune_group(self, launches: int = 3) -> int:
+        """The autotuner of the force schedule: lanes per neighbor row (1 = a lane per atom,
+        up to 32), each timed with HIP events on this frame; the fastest wins. Syncs."""
+        best, best_time = 16, math.inf
+        for group in _GROUPS:
+            self._neighbors.native.group = group
+            self._evaluate(virial=False)
+            start = torch.cuda.Event(enable_timing=True)
+            end = torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(launches):
+                self._evaluate(virial=False)
+            end.record()
+            torch.cuda.synchronize(self._device)
+            elapsed = start.elapsed_time(end)
+            if elapsed < best_time:
+                best, best_time = group, elapsed
+        self._neighbors.native.group = best
+        self.tuning = {"group": best, "ms_per_evaluation": best_time / launches}
+        return best
+
+    def _settle(self) -> None:
+        _native.call("md_settle", _native.dtype_tag(self._dtype),
+                     _native.ctypes.byref(self._system), _native.stream(self._device))
+
+    def _capture(self) -> torch.cuda.CUDAGraph:
+        graph = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize(self._device)
+        with torch.cuda.graph(graph):
+            self._launch_step()
+        torch.cuda.synchronize(self._device)
+        return graph
+
+    def _to_atoms(self, tensor: Tensor) -> Tensor:
+        """A slot-ordered per-atom tensor in atom order."""
+        out = torch.empty_like(tensor)
+        width = tensor[0].numel() if tensor.dim() > 1 else 1
+        _native.call("md_to_atom_order", tensor.element_size(), _native.ptr(tensor),
+                     _native.ptr(self._neighbors.buffers["ids"]),
+                     _native.ctypes.c_int64(self._num_atoms), width, _native.ptr(out),
+                     _native.stream(self._device))
+        return out
+
+    def run(self, steps: Integral, *, sample_every: Integral | None = None, echo=False,
+            sample_file=None):
+        """Advance ``steps`` steps (``md/_simulation.py:198-272``).
+
+        With ``sample_every``, returns the panel at each step that is a multiple of it, as host
+        tensors under ``FIELDS`` (and ``conserved_energy`` for NVT) plus ``step``. ``echo``
+        prints each sample. HDF5 sample files are not supported (h5py is not part of this
+        build). Reads the neighbor status every 1,000 steps and at the end.
+        """
+        steps = _count("steps", steps)
+        if sample_file is not None:
+            raise NotImplementedError("HDF5 sample files are not supported")
+        if sample_every is not None and _count("sample_every", sample_every) == 0:
+            raise ValueError("sample_every must be positive")
+        if echo is not False and sample_every is None:
+            raise ValueError("echo needs sample_every")
+        self._require()
+        start = self._step
+        rows, due = [], []
+        stream = sys.stdout if echo is True else echo
+        for done in range(1, steps + 1):
+            self._graph.replay()
+            step = start + done
+            if sample_every is not None and step % sample_every == 0:
+                values = self.sample()
+                rows.append([values[name] for name in self.fields])
+                due.append(step)
+                if stream:
+                    stream.write(f"{step:>10d} " + " ".join(f"{v:14.6f}" for v in rows[-1]) + "\n")
+            if done % _STATUS_INTERVAL == 0 or done == steps:
+                self._check(step)
+        self._step += steps
+        if sample_every is None:
+            return None
+        table = torch.tensor(rows, dtype=torch.float64) if rows else torch.zeros((0, len(self.fields)))
+        samples = {"step": torch.tensor(due, dtype=torch.int64)}
+        samples.update({name: table[:, k] for k, name in enumerate(self.fields)})
+        return samples
+
+    @property
+    def fields(self) -> tuple[str, ...]:
+        return FIELDS + (("conserved_energy",) if self._nvt else ())
+
+    @property
+    def model(self) -> Potential:
+        return self._model
+
+    @property
+    def masses(self) -> Tensor:
+        return self._masses.clone()
+
+    def sample(self) -> dict[str, float]:
+        """The current frame's panel (potential, kinetic, total energy; T; P; NVT: conserved
+        energy). Syncs with the host."""
+        self._require()
+        self._settle()
+        self._evaluate(virial=True)
+        masses = self.slot_masses.double()
+        kinetic = float(0.5 * (masses[:, None] * self.velocities.double().square()).sum())
+        energy = float(self.energy)
+        volume = float(self.cell.double().prod())
+        values = {
+            "potential_energy": energy,
+            "kinetic_energy": kinetic,
+            "total_energy": energy + kinetic,
+            "temperature": 2.0 * kinetic / (self._degrees_of_freedom * BOLTZMANN),
+            "pressure": (2.0 * kinetic + float(self.virial.double().trace())) / (3.0 * volume),
+        }
+        if self._nvt:
+            q, p = self.thermostat[0:3].double(), self.thermostat[3:6].double()
+            bath = float((p.square() / (2.0 * self.chain_masses)).sum()
+                         + self._degrees_of_freedom * self._kT * q[0] + self._kT * q[1:].sum())
+            values["conserved_energy"] = energy + kinetic + bath
+        return values
+
+    def state(self) -> State:
+        """A copy of the current frame (velocities with the pending chain factor applied)."""
+        self._require()
+        self._settle()
+        chain = None
+        if self._nvt:
+            chain = ChainState(self.thermostat[0:3].clone(), self.thermostat[3:6].clone())
+        return State(self._to_atoms(self.positions), self._to_atoms(self.velocities),
+                     self._to_atoms(self.forces), self.cell.clone(), self._to_atoms(self.images),
+                     self._step, chain)
+
+    def _check(self, step: int) -> None:
+        try:
+            self._neighbors._check_status()
+        except ValueError as error:
+            self._failure = f"by step {step}: {error}"
+            raise ValueError(f"the simulation failed {self._failure}") from error
+
+    def _require(self) -> None:
+        if self._failure is not None:
+            raise RuntimeError(
+                f"the simulation failed {self._failure}; start a new one from an earlier state()",
+            )
+
+
+def nve(model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor, cell: Tensor, *,
+        timestep: Real, skin: Real = 0.1, capacity_multiplier: Real = 1.25,
+        images: Tensor | None = None, step: Integral = 0) -> Simulation:
+    """Constant energy by velocity Verlet (``md/_ensembles/_nve.py:70-107``)."""
+    return Simulation(model, positions, velocities, masses, cell, timestep=timestep, skin=skin,
+                      capacity_multiplier=capacity_multiplier, images=images, step=step)
+
+
+def nvt(model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor, cell: Tensor, *,
+        timestep: Real, temperature: Real, thermostat_timescale: Real, skin: Real = 0.1,
+        capacity_multiplier: Real = 1.25, images: Tensor | None = None, step: Integral = 0,
+        thermostat: ChainState | None = None) -> Simulation:
+    """Constant temperature by velocity Verlet and a Nose-Hoover chain of 3
+    (``md/_ensembles/_nvt.py:132-179``): chain masses g kT tau^2, kT tau^2, kT tau^2."""
+    for name, value in (("temperature", temperature), ("thermostat_timescale", thermostat_timescale)):
+        _positive(name, value)
+    return Simulation(model, positions, velocities, masses, cell, timestep=timestep, skin=skin,
+                      capacity_multiplier=capacity_multiplier, images=images, step=step,
+                      temperature=temperature, thermostat_timescale=thermostat_timescale,
+                      thermostat=thermostat)
+
+
+__all__ = ("FIELDS", "Simulation", "nve", "nvt")

@@ -387,14 +387,14 @@ class NeighborList:
         return int(self._reuse.rebuild_count.item())
 
     def _max_moved(self) -> float:
-        """The largest squared displacement the last reuse check measured, in nm^2.
+        """The largest squared displacement the last reuse check measured, in nm^2: the maximum
+        of the 64 per-block slots after the first 64 bytes of scan_temp. Syncs with the host."""
+        import struct
 
-        Syncs with the host.
-        """
-        raw = self._buffers.scan_temp[8:16].view(torch.int64).item()
+        raw = int(self._buffers.scan_temp[64:64 + 8 * 64].view(torch.int64).max().item())
         if self._buffers.sorted_positions.dtype == torch.float32:
-            return float(torch.tensor([raw & 0xFFFFFFFF], dtype=torch.int64).to(torch.int32).view(torch.float32))
-        return float(torch.tensor([raw], dtype=torch.int64).view(torch.float64))
+            return struct.unpack("<f", struct.pack("<I", raw & 0xFFFFFFFF))[0]
+        return struct.unpack("<d", struct.pack("<q", raw))[0]
 
     def _live_buffers(self) -> _Buffers:
         if self._buffers is None:
