@@ -255,6 +255,73 @@ int ndn_md_settle(int dtype, const ndn_md_system* system, ndn_stream_t stream);
 int ndn_md_to_atom_order(int element_bytes, const void* in, const int32_t* ids, int64_t num_atoms,
                          int width, void* out, ndn_stream_t stream);
 
+/* ---- many systems in one launch ---------------------------------------------------------- */
+
+/* A batch of S independent systems (one atom type each), N atoms in all. Every system owns a
+ * contiguous range of atoms (atom_start) and of one global bin space (bin_start, its sub-bins
+ * dims [3] of widths [3], each at least cutoff / span wide). Its potential is LJ (kinds = NDN_LJ,
+ * lj_rows [4] = sigma^2, 4 eps, 24 eps, the energy at the cutoff) or a padded table row
+ * (table_of >= 0 into tables [R, K + 2], table_start [R], table_inverse_spacing [R]). The rest
+ * is scratch (bin_counts zero between calls) and outputs: forces [N, 3], atom_terms [N, 7],
+ * energy [S], virial [S, 3, 3]. Positions may lie outside their cells. No reference
+ * counterpart: the reference evaluates one system per call. */
+typedef struct ndn_batch {
+    int64_t num_systems;
+    int64_t num_atoms;
+    const void* positions;
+    const int32_t* system_of;
+    const int64_t* atom_start;
+    const int32_t* bin_start;
+    const int32_t* dims;
+    const void* widths;
+    const void* cells;
+    const void* cutoffs;
+    const int32_t* kinds;
+    const void* lj_rows;
+    const int32_t* table_of;
+    const void* tables;
+    const void* table_start;
+    const void* table_inverse_spacing;
+    int64_t knots;
+    int64_t total_bins;
+    int32_t span;
+    int32_t reserved;
+    int32_t* bin_of;
+    int32_t* bin_counts;
+    int32_t* bin_offsets;
+    int32_t* unordered;
+    int32_t* order;
+    void* wrapped4;
+    void* sorted4;
+    uint8_t* scan_temp;
+    int64_t scan_temp_bytes;
+    int32_t* status;
+    void* forces;
+    void* atom_terms;
+    void* energy;
+    void* virial;
+} ndn_batch;
+
+/* scan_temp bytes of a batch of `total_bins` bins. */
+int64_t ndn_batch_scan_bytes(int64_t total_bins);
+
+/* E, F, W of every system of the batch: bin, sort, and one force launch over all atoms with no
+ * stored list; schedule 0 = a lane per atom, 1 = a wave per atom. */
+int ndn_batch_efw(int dtype, const ndn_batch* batch, int schedule, ndn_stream_t stream);
+
+/* Workspace bytes of ndn_batch_table_vjp. */
+int64_t ndn_batch_vjp_bytes(int64_t num_atoms, int64_t knots);
+
+/* The hand-written backward of a batch of tabulated systems sharing one table (K knots): with the
+ * batch binned by a preceding ndn_batch_efw, and upstream gradients gE [S], gF [N, 3], gW [S, 3, 3]
+ * of its E, F and W, writes grad_table [K] = sum over systems of gE dE/dtable + gF . dF/dtable
+ * + gW : dW/dtable (exact: E, F and W are linear in the knots; the last knot gets 0). Replaces the
+ * reference's torch.func double backward over vmapped frames (training/_loss.py:71-125,
+ * training/_fit.py:245-271) for the table's gradient. */
+int ndn_batch_table_vjp(int dtype, const ndn_batch* batch, const void* grad_energy,
+                        const void* grad_forces, const void* grad_virial, void* workspace,
+                        int64_t workspace_bytes, void* grad_table, ndn_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

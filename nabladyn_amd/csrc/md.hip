@@ -680,7 +680,9 @@ __global__ void __launch_bounds__(kBlock) force_kernel(StepArgs<T> a, Pot pot, T
     if (live)
     {
         c = pos[row];
-        count = a.counts[row];
+        // a row that overflowed holds only `stride` entries (the status bit reports it)
+        const index_t listed = a.counts[row];
+        count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
     }
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const T low = cutoff + T(1e-3);
@@ -965,6 +967,8 @@ void launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff, T c
     }
 }
 
+constexpr int kMaxGroup = 32;
+
 int valid_group(int group)
 {
     return group == 1 || group == 4 || group == 8 || group == 16 || group == 32;
@@ -972,8 +976,12 @@ int valid_group(int group)
 
 template <typename T>
 int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coefficients, int group,
-                  bool virial, bool kick, double* partials, hipStream_t stream)
+                  bool virial, bool kick, double* partials, std::int64_t partial_slots,
+                  hipStream_t stream)
 {
+    NDN_REQUIRE(valid_group(group), "group must be 1, 4, 8, 16 or 32");
+    NDN_REQUIRE(static_cast<std::int64_t>(xcd_grid(force_blocks(a.num_atoms, group))) <= partial_slots,
+                "the force launch has more blocks than partial slots");
     const T cutoff = static_cast<T>(pot->cutoff);
     const T cutoff_squared = static_cast<T>(pot->cutoff * pot->cutoff);
     const T half_timestep = static_cast<T>(0.5 * a.timestep);
@@ -1011,7 +1019,7 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream);
     if (rebuilt != NDN_OK) return rebuilt;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, settings->virial != 0,
-                                        true, s.partials, stream);
+                                        true, s.partials, s.partial_slots, stream);
     if (forces != NDN_OK) return forces;
     const int blocks = static_cast<int>(xcd_grid(force_blocks(s.num_atoms, m.group)));
     finish_step_kernel<T><<<1, kBlock, 0, stream>>>(a, s.partials, blocks, 1);
@@ -1029,7 +1037,7 @@ int md_evaluate_impl(const ndn_md_system& s, const ndn_potential* pot, const ndn
     const int prepared = prepare_potential<T>(pot, s.coefficients, stream, &view);
     if (prepared != NDN_OK) return prepared;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, virial != 0, kick != 0,
-                                        s.partials, stream);
+                                        s.partials, s.partial_slots, stream);
     if (forces != NDN_OK) return forces;
     const int blocks = static_cast<int>(xcd_grid(force_blocks(s.num_atoms, m.group)));
     finish_step_kernel<T><<<1, kBlock, 0, stream>>>(a, s.partials, blocks, 0);
@@ -1057,8 +1065,9 @@ int check_system(const ndn_md_system* s, const ndn_md_list* m)
     NDN_REQUIRE(m->stride == 0 || m->index != nullptr, "a list with rows needs its index");
     NDN_REQUIRE(m->stride >= 0 && m->stride * s->num_atoms <= (1LL << 40), "bad stride");
     NDN_REQUIRE(valid_group(m->group), "group must be 1, 4, 8, 16 or 32");
-    NDN_REQUIRE(s->partial_slots >= static_cast<std::int64_t>(xcd_grid(force_blocks(s->num_atoms, 1))),
-                "too few partial slots");
+    NDN_REQUIRE(s->partial_slots >=
+                    static_cast<std::int64_t>(xcd_grid(force_blocks(s->num_atoms, m->group))),
+                "too few partial slots for this force schedule");
     return NDN_OK;
 }
 
@@ -1075,7 +1084,8 @@ int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     NDN_REQUIRE(num_atoms >= 1 && partial_slots && coefficient_bytes_out, "bad arguments");
-    *partial_slots = std::max<std::int64_t>(xcd_grid(force_blocks(num_atoms, 1)), 8);
+    // the most blocks any schedule launches: 32 lanes per row, 8 rows per block
+    *partial_slots = std::max<std::int64_t>(xcd_grid(force_blocks(num_atoms, kMaxGroup)), 8);
     *coefficient_bytes_out = coefficient_bytes(pot, dtype == NDN_F32 ? 4 : 8);
     return NDN_OK;
 }
@@ -1155,13 +1165,13 @@ int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* po
         StepArgs<float> a = step_args<float>(*system, nullptr, nullptr, *md_list);
         a.timestep = timestep;
         return launch_forces<float>(a, pot, system->coefficients, md_list->group, virial != 0,
-                                    kick != 0, system->partials, stream);
+                                    kick != 0, system->partials, system->partial_slots, stream);
     }
     NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     StepArgs<double> a = step_args<double>(*system, nullptr, nullptr, *md_list);
     a.timestep = timestep;
     return launch_forces<double>(a, pot, system->coefficients, md_list->group, virial != 0,
-                                 kick != 0, system->partials, stream);
+                                 kick != 0, system->partials, system->partial_slots, stream);
 }
 
 int ndn_md_settle(int dtype, const ndn_md_system* system, ndn_stream_t stream)

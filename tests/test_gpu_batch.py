@@ -1,0 +1,129 @@
+"""The batched engine: every system's E, F, W against the fp64 oracle and the plain per-system
+path, for every schedule; each system's results identical alone and in the mix.
+
+Tolerance (fp32): max|dF| / max|F_ref| and |dW| / max|W_ref| against fp64 on the same fp32
+positions within max(1e-5, 1.5 x the distance of the reference's own fp32 evaluation — its
+autograd restatement run in fp32 — from the same fp64 values). For tables that distance reaches
+2.5e-5 on these systems (the knot coordinate (r - start) / h loses fp32 bits, and net forces are
+small differences of large pair forces); for LJ it stays below 1e-5. fp64: 1e-12.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from nabladyn_amd import batch, potential, workloads
+from oracle import neighbors as on
+from oracle import potentials as op
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(device, dtype, count=24, high=4096, seed=2):
+    raw = workloads.ragged_systems(count, high=high, seed=seed, min_length=2.45)
+    table = workloads.lj_table()
+    out = []
+    for sysd in raw:
+        n = sysd["positions"].shape[0]
+        x = torch.tensor(sysd["positions"], dtype=dtype, device=device)
+        cell = torch.tensor(sysd["cell"], dtype=dtype, device=device)
+        types = torch.zeros(n, dtype=torch.int64, device=device)
+        if sysd["kind"] == "lj":
+            model = potential.lennard_jones(
+                atom_types=types, sigma=torch.tensor([0.34], dtype=dtype, device=device),
+                epsilon=torch.tensor([0.996], dtype=dtype, device=device), cutoff=1.0)
+        else:
+            model = potential.tabulated(
+                atom_types=types, table=torch.tensor(table[None], dtype=dtype, device=device),
+                start=torch.tensor([0.3], dtype=dtype, device=device), cutoff=1.2)
+        out.append((x, cell, model))
+    return out
+
+
+def _reference(x, cell, model):
+    """fp64 oracle E, F, W on the system's (rounded) positions."""
+    xn = x.cpu().numpy()
+    cn = cell.cpu().numpy()
+    xw = on.wrap(xn, cn)[0].astype(np.float64)
+    cell64 = cn.astype(np.float64)
+    pairs = on.brute_pairs(xw, cell64, model.cutoff).T
+    if isinstance(model, potential.LennardJones):
+        return op.lj_efw(xw, cell64, pairs, np.zeros(len(xw), np.int64),
+                         model.sigma.detach().cpu().numpy(), model.epsilon.detach().cpu().numpy(),
+                         model.cutoff)
+    return op.table_efw(xw, cell64, pairs, np.zeros(len(xw), np.int64),
+                        model.table.detach().cpu().numpy(), model.start.cpu().numpy(),
+                        model.cutoff, spacing=model.spacing.cpu().numpy())
+
+
+def _reference_fp32(x, cell, model):
+    """The reference's autograd route (``quantity.py:93-117``) evaluated in fp32: F, W."""
+    xw = on.wrap(x.cpu().numpy(), cell.cpu().numpy())[0]
+    pairs = torch.tensor(on.brute_pairs(xw.astype(np.float64), cell.cpu().numpy().astype(np.float64),
+                                        model.cutoff).T).long()
+    types = torch.zeros(len(xw), dtype=torch.int64)
+    f32 = torch.float32
+    if isinstance(model, potential.LennardJones):
+        sigma, eps = model.sigma.detach().cpu().to(f32), model.epsilon.detach().cpu().to(f32)
+        energy = lambda p, s, c, i: op.t_lj_energy(p, s, c, i, types, sigma, eps, model.cutoff)
+    else:
+        args = (model.table.detach().cpu().to(f32), model.start.cpu().to(f32),
+                model.spacing.cpu().to(f32))
+        energy = lambda p, s, c, i: op.t_table_energy(p, s, c, i, types, *args, model.cutoff)
+    _, forces, virial = op.torch_efw(energy, torch.tensor(xw, dtype=f32),
+                                     cell.cpu().to(f32), pairs)
+    return forces.numpy(), virial.numpy()
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-30))
+
+
+@pytest.mark.parametrize("schedule", batch.SCHEDULES)
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float64, 1e-12)])
+def test_batch_matches_oracle(device, schedule, dtype, tol):
+    systems = _systems(device, dtype, count=12 if dtype == torch.float64 else 24)
+    b = batch.Batch(systems, schedule=schedule)
+    energy, forces, virial = b.evaluate()
+    for k, (f, (x, cell, model)) in enumerate(zip(b.split(forces), systems)):
+        e0, f0, w0 = _reference(x, cell, model)
+        tol_f = tol_w = tol
+        if dtype == torch.float32:
+            f32, w32 = _reference_fp32(x, cell, model)
+            tol_f = max(tol, 1.5 * _rel(f32, f0))
+            tol_w = max(tol, 1.5 * _rel(w32, w0))
+        assert _rel(f.cpu().numpy(), f0) <= tol_f, (k, schedule)
+        assert _rel(virial[k].cpu().numpy(), w0) <= tol_w, (k, schedule)
+        assert abs(float(energy[k]) - e0) <= tol * max(1.0, abs(e0)) * 10, (k, schedule)
+
+
+def test_batch_matches_plain_path(device):
+    systems = _systems(device, torch.float32)
+    b = batch.Batch(systems, schedule=("wave", 2))
+    energy, forces, virial = b.evaluate()
+    plain = batch.plain_efw(systems)
+    for k, (f, (e1, f1, w1)) in enumerate(zip(b.split(forces), plain)):
+        assert _rel(f.cpu(), f1.cpu()) <= 2e-5, k
+        assert _rel(virial[k].cpu(), w1.cpu()) <= 2e-5, k
+
+
+@pytest.mark.parametrize("schedule", [("wave", 2), ("thread", 1)])
+def test_alone_equals_in_the_mix(device, schedule):
+    systems = _systems(device, torch.float32)
+    mix = batch.Batch(systems, schedule=schedule)
+    e_mix, f_mix, w_mix = (t.clone() for t in mix.evaluate())
+    f_mix = mix.split(f_mix)
+    for k in (0, 5, 11, 23):
+        alone = batch.Batch([systems[k]], schedule=schedule)
+        e, f, w = alone.evaluate()
+        assert torch.equal(e[0], e_mix[k])
+        assert torch.equal(f, f_mix[k])
+        assert torch.equal(w[0], w_mix[k])
+
+
+def test_autotuner_picks_a_schedule(device):
+    systems = _systems(device, torch.float32, count=16)
+    b = batch.Batch(systems)
+    b.evaluate()
+    assert b.chosen in batch.SCHEDULES
+    assert set(b.timings) == set(batch.SCHEDULES)

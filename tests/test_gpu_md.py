@@ -109,8 +109,11 @@ def test_trajectory_matches_oracle(device, dtype, steps, tol, ensemble):
     f_err = float((state.forces.cpu().double() - ref.forces.double()).abs().max())
     assert f_err / float(ref.forces.abs().max()) <= 10 * tol
     if ensemble == "nvt":
-        assert torch.allclose(state.thermostat.positions.cpu(), ref.chain_x, rtol=1e-6, atol=1e-12)
-        assert torch.allclose(state.thermostat.momenta.cpu(), ref.chain_p, rtol=1e-5, atol=1e-10)
+        # K is summed in fp64 here and in the frame's dtype by the reference's eager torch, so in
+        # fp32 the chain's driving term differs at the 1e-7 level
+        rtol, atol = (1e-8, 1e-12) if dtype == torch.float64 else (1e-3, 1e-6)
+        assert torch.allclose(state.thermostat.positions.cpu(), ref.chain_x, rtol=rtol, atol=atol)
+        assert torch.allclose(state.thermostat.momenta.cpu(), ref.chain_p, rtol=rtol, atol=atol)
 
 
 def test_nvt_conserved_energy_band_argon(argon, device):
