@@ -6,10 +6,13 @@ at 120 K with a 0.1 ps timescale, 4 fs steps; each step one HIP graph replay wit
 decided on the device. Synthetic, random-seeded inputs (no datasets).
 
 Prints one JSON line: atom-steps/s over K timed steps (after W untimed ones), the force
-kernel's roofline (algorithmic bytes / its event-timed average duration, against 8 TB/s), and
-a CPU baseline (the oracle's restatement of the reference path on this host's cores).
+kernel's roofline (algorithmic bytes / its event-timed average duration, against 8 TB/s), the
+batched engine's frames/s against the plain per-system path (256 ragged systems), the serving
+loop's p50/p99 latency and throughput against the plain per-request path, and a CPU baseline
+(the oracle's restatement of the reference path on this host's cores).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batching]
+                    [--no-serving]
 """
 
 import argparse
@@ -84,6 +87,73 @@ def time_force_kernel(torch, sim, launches=50):
     return start.elapsed_time(end) / 1e3 / launches
 
 
+def bench_batching(torch, count=256, repeats=10):
+    """256 ragged systems (log-uniform 64..32768 atoms, aspect ratios 1..2, half LJ, half a
+    200-knot table; edges stretched to >= 2.45 nm where 21 nm^-3 would make them shorter than
+    twice the table's cutoff), fp32 E/F/W: the batch engine (one set of launches, schedule
+    autotuned) against the plain per-system path (NeighborList + energy_force_virial each)."""
+    from nabladyn_amd import batch, workloads
+
+    device = torch.device("cuda", 0)
+    raw = workloads.ragged_systems(count, seed=2, min_length=2.45)
+    systems = batch.make_systems(raw, device)
+    b = batch.Batch(systems, schedule="auto")
+    b.evaluate()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(repeats):
+        b.launch(b.chosen)
+    end.record()
+    torch.cuda.synchronize()
+    t_batch = start.elapsed_time(end) / 1e3 / repeats
+    batch.plain_efw(systems[:4])  # warm the plain path's launches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.plain_efw(systems)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    atoms = sum(int(x.shape[0]) for x, _, _ in systems)
+    return {
+        "systems": count,
+        "atoms": atoms,
+        "schedule": list(b.chosen),
+        "batched_frames_per_s": count / t_batch,
+        "plain_frames_per_s": count / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "batched_ms_per_set": t_batch * 1e3,
+        "plain_ms_per_set": t_plain * 1e3,
+    }
+
+
+def bench_serving(torch, count=64, qps=40.0):
+    """A Poisson stream of `count` requests at `qps` (70% E/F/W, 30% 500-step NVT segments,
+    256..262144 atoms log-uniform, LJ or 200-knot table, fp32): the continuous-batching server's
+    p50/p99 latency and throughput, and the throughput of the same requests all queued at once
+    on the server and on the plain per-request path."""
+    from nabladyn_amd import serve, workloads
+
+    device = torch.device("cuda", 0)
+    specs = workloads.request_stream(count, qps, seed=4)
+    live = serve.Server(device).run(serve.make_requests(specs, device))
+    queued = [dict(spec, arrival=0.0) for spec in specs]
+    capacity = serve.Server(device).run(serve.make_requests(queued, device))
+    plain = serve.serve_plain(serve.make_requests(queued, device))
+    return {
+        "requests": count,
+        "qps": qps,
+        "md_requests": sum(spec["kind"] == "md" for spec in specs),
+        "p50_latency_ms": live["p50_latency_ms"],
+        "p99_latency_ms": live["p99_latency_ms"],
+        "atom_steps_per_s": live["atom_steps_per_s"],
+        "capacity_atom_steps_per_s": capacity["atom_steps_per_s"],
+        "plain_atom_steps_per_s": plain["atom_steps_per_s"],
+        "speedup_vs_plain": capacity["atom_steps_per_s"] / plain["atom_steps_per_s"],
+        "plain_p50_latency_ms": plain["p50_latency_ms"],
+        "plain_p99_latency_ms": plain["p99_latency_ms"],
+    }
+
+
 def cpu_baseline(x, cell, steps_per_rebuild):
     """The oracle (the reference's eager torch force path + C cell list) on this host's cores:
     one forced rebuild and one force evaluation plus the Verlet/chain update of the same
@@ -140,6 +210,13 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=2000)
     parser.add_argument("--warmup", type=int, default=200)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-batching", action="store_true")
+    parser.add_argument(
+        "--kernel", choices=("force", "copy"),
+        help="only launch one kernel eagerly, for counter collection: 'force' builds the "
+             "2M-atom system and launches its force kernel 20 times; 'copy' copies 1 GiB on the "
+             "device 20 times (the scale of the traffic counters)")
+    parser.add_argument("--no-serving", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(
@@ -153,8 +230,24 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
+    if args.kernel == "copy":
+        source = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        target = torch.empty_like(source)
+        source.fill_(1.0)
+        for _ in range(20):
+            target.copy_(source)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": "copy", "bytes_read": source.numel() * 4,
+                          "bytes_written": source.numel() * 4, "launches": 20}))
+        return 0
     log("building the 2,097,152-atom system ...")
     sim, x, cell = build_simulation(torch)
+    if args.kernel == "force":
+        seconds = time_force_kernel(torch, sim, launches=20)
+        pairs = sim._neighbors.num_pairs
+        print(json.dumps({"kernel": "force", "launches": 23, "ms": seconds * 1e3,
+                          "pairs_within_radius": pairs, "group": sim.group}))
+        return 0
     n = x.shape[0]
     torch.cuda.synchronize()
     log(f"warm-up {args.warmup} steps ...")
@@ -235,6 +328,12 @@ def main() -> int:
             "frac": step_bytes * args.steps / elapsed / HBM_PEAK,
         },
     }
+    if not args.no_batching:
+        log("batching: 256 ragged systems ...")
+        result["batching"] = bench_batching(torch)
+    if not args.no_serving:
+        log("serving: a Poisson request stream ...")
+        result["serving"] = bench_serving(torch)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle on the host cores) ...")
         del sim

@@ -182,8 +182,8 @@ typedef struct ndn_list_state {
 
 /* The MD step's own neighbor structure: sub-bins at least radius / 2 wide (dims fixed for the
  * run), and a full list (both orientations of every pair within the radius) in fixed-stride rows,
- * one per slot, holding neighbor slots. `group` lanes of a wave read one row in the force kernel
- * (1, 4, 8, 16 or 32). status [2] latches errors as the neighbor list's does, with
+ * one per slot, holding neighbor slots; stride is a multiple of 4 (16-byte row reads). `group`
+ * lanes of a wave read one row in the force kernel (2, 4, 8 or 16). status [2] latches errors as the neighbor list's does, with
  * NDN_STATUS_FULL_OVERFLOW when a row needs more than `stride` entries (status[1] = the width
  * needed). scan_temp holds a 1024-byte header and the scan tiles. */
 typedef struct ndn_md_list {

@@ -245,6 +245,33 @@ class Batch:
         torch.cat([x.detach().to(self.dtype) for x in positions], out=self.positions)
 
 
+def make_systems(raw, device, dtype=torch.float32, table=None):
+    """(positions, cell, model) for each dict of ``workloads.ragged_systems``: argon LJ
+    (sigma 0.34 nm, epsilon 0.996 kJ/mol, cutoff 1.0 nm) or the 200-knot table on
+    [0.3, 1.2] nm (``workloads.lj_table``), one atom type."""
+    from . import potential, workloads
+
+    table = workloads.lj_table() if table is None else table
+    out = []
+    for system in raw:
+        n = system["positions"].shape[0]
+        x = torch.tensor(system["positions"], dtype=dtype, device=device)
+        cell = torch.tensor(system["cell"], dtype=dtype, device=device)
+        types = torch.zeros(n, dtype=torch.int64, device=device)
+        if system["kind"] == "lj":
+            model = potential.lennard_jones(
+                atom_types=types,
+                sigma=torch.tensor([workloads.ARGON_SIGMA], dtype=dtype, device=device),
+                epsilon=torch.tensor([workloads.ARGON_EPSILON], dtype=dtype, device=device),
+                cutoff=1.0)
+        else:
+            model = potential.tabulated(
+                atom_types=types, table=torch.tensor(table[None], dtype=dtype, device=device),
+                start=torch.tensor([0.3], dtype=dtype, device=device), cutoff=1.2)
+        out.append((x, cell, model))
+    return out
+
+
 def plain_efw(systems):
     """The per-system path: a zero-skin NeighborList and quantity.energy_force_virial per system
     (the reference's way of evaluating one frame). Returns [(E, F, W)]."""

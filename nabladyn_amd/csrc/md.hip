@@ -112,27 +112,85 @@ __device__ __forceinline__ double reciprocal<double>(double value)
     return 1.0 / value;
 }
 
-// LJ with the hardware reciprocal (1 ulp in fp32): the force kernel's potential.
+// The force kernel's potentials. A pair's two types arrive as the w of the two gathered
+// float4s (small integers, exact); bind() runs once per kernel and returns what the pair loop
+// evaluates: eval(type_i, type_j, r2, u, fs) with fs = -(du/dr)/r.
+//
+// LJ with one type: the row's coefficients held in registers for the whole kernel; the hardware
+// reciprocal (1 ulp in fp32).
 template <typename T>
-struct FastLJ
+struct OneTypeLJ
 {
+    template <typename E>
+    static constexpr int waves = sizeof(E) == 4 ? 8 : 1;
+
     const LJRow<T>* __restrict__ rows;
-    const std::int64_t* __restrict__ types;
+
+    struct Bound
+    {
+        T sigma_squared, four_eps, twentyfour_eps, shift;
+
+        __device__ __forceinline__ void eval(T, T, T r2, T& u, T& fs) const
+        {
+            const T inverse = reciprocal(r2);
+            const T s = sigma_squared * inverse;
+            const T p6 = s * s * s;
+            const T p12 = p6 * p6;
+            u = four_eps * (p12 - p6) - shift;
+            fs = twentyfour_eps * ((p12 + p12) - p6) * inverse;
+        }
+    };
+
+    __device__ __forceinline__ Bound bind() const
+    {
+        const LJRow<T> c = rows[0];
+        return Bound{c.sigma_squared, c.four_eps, c.twentyfour_eps, c.shift};
+    }
+};
+
+// LJ with several types: the pair's row read from the small, cache-resident coefficient table.
+template <typename T>
+struct TypedLJ
+{
+    template <typename E>
+    static constexpr int waves = sizeof(E) == 4 ? 7 : 1;
+
+    const LJRow<T>* __restrict__ rows;
     int num_types;
 
-    __device__ __forceinline__ int row(std::int64_t i, std::int64_t j) const
-    {
-        return num_types == 1 ? 0 : pair_row(types[i], types[j], num_types);
-    }
+    __device__ __forceinline__ const TypedLJ& bind() const { return *this; }
 
-    __device__ __forceinline__ void eval(int row_index, T r2, T& u, T& fs) const
+    __device__ __forceinline__ void eval(T type_i, T type_j, T r2, T& u, T& fs) const
     {
-        const LJRow<T> c = rows[row_index];
+        const LJRow<T> c = rows[pair_row(static_cast<int>(type_i), static_cast<int>(type_j),
+                                         num_types)];
         const T inverse = reciprocal(r2);
         const T s = c.sigma_squared * inverse;
         const T p6 = s * s * s;
-        u = c.four_eps * (p6 * p6 - p6) - c.shift;
-        fs = c.twentyfour_eps * (T(2) * p6 * p6 - p6) * inverse;
+        const T p12 = p6 * p6;
+        u = c.four_eps * (p12 - p6) - c.shift;
+        fs = c.twentyfour_eps * ((p12 + p12) - p6) * inverse;
+    }
+};
+
+// The Catmull-Rom table, its row from the carried types.
+template <typename T>
+struct TypedTable
+{
+    template <typename E>
+    static constexpr int waves = sizeof(E) == 4 ? 5 : 1;
+
+    Table<T> table;
+
+    __device__ __forceinline__ const TypedTable& bind() const { return *this; }
+
+    __device__ __forceinline__ void eval(T type_i, T type_j, T r2, T& u, T& fs) const
+    {
+        const int row = table.num_types == 1
+                            ? 0
+                            : pair_row(static_cast<int>(type_i), static_cast<int>(type_j),
+                                       table.num_types);
+        table.eval(row, r2, u, fs);
     }
 };
 
@@ -320,7 +378,7 @@ __global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T tim
         p4.x = out[0];
         p4.y = out[1];
         p4.z = out[2];
-        p4.w = T(0);
+        p4.w = a.types ? static_cast<T>(a.types[atom]) : T(0);
         reinterpret_cast<V4*>(a.positions4)[atom] = p4;
         if (crossed)
         {
@@ -368,24 +426,28 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) md_assign_kernel(StepArgs<T> a, int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= a.num_atoms) return;
-    const T* p = a.positions + 3 * slot;
-    const index_t bin = sub_bin(p[0], a.wx, a.nx) +
-                        a.nx * (sub_bin(p[1], a.wy, a.ny) + a.ny * sub_bin(p[2], a.wz, a.nz));
-    a.bin_of[slot] = bin;
-    atomicAdd(a.bin_counts + bin, 1);
+    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    {
+        const T* p = a.positions + 3 * slot;
+        const index_t bin = sub_bin(p[0], a.wx, a.nx) +
+                            a.nx * (sub_bin(p[1], a.wy, a.ny) + a.ny * sub_bin(p[2], a.wz, a.nz));
+        a.bin_of[slot] = bin;
+        atomicAdd(a.bin_counts + bin, 1);
+    }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) md_scatter_kernel(StepArgs<T> a, int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= a.num_atoms) return;
-    const index_t bin = a.bin_of[slot];
-    a.unordered[a.bin_offsets[bin] + atomicSub(a.bin_counts + bin, 1) - 1] =
-        static_cast<index_t>(slot);
+    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    {
+        const index_t bin = a.bin_of[slot];
+        a.unordered[a.bin_offsets[bin] + atomicSub(a.bin_counts + bin, 1) - 1] =
+            static_cast<index_t>(slot);
+    }
 }
 
 // One thread per sub-bin (a handful of atoms): ranks each entry by atom id, so the new order is
@@ -394,17 +456,19 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) md_order_kernel(StepArgs<T> a, int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t bin = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (bin >= a.bins) return;
-    const index_t begin = a.bin_offsets[bin];
-    const index_t end = a.bin_offsets[bin + 1];
-    for (index_t i = begin; i < end; ++i)
+    for (std::int64_t bin = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         bin < a.bins; bin += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
     {
-        const index_t slot = a.unordered[i];
-        const index_t id = a.ids[slot];
-        index_t rank = 0;
-        for (index_t k = begin; k < end; ++k) rank += a.ids[a.unordered[k]] < id;
-        a.order[begin + rank] = slot;
+        const index_t begin = a.bin_offsets[bin];
+        const index_t end = a.bin_offsets[bin + 1];
+        for (index_t i = begin; i < end; ++i)
+        {
+            const index_t slot = a.unordered[i];
+            const index_t id = a.ids[slot];
+            index_t rank = 0;
+            for (index_t k = begin; k < end; ++k) rank += a.ids[a.unordered[k]] < id;
+            a.order[begin + rank] = slot;
+        }
     }
 }
 
@@ -424,22 +488,24 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) md_permute_kernel(StepArgs<T> a, int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= a.num_atoms) return;
-    const std::int64_t old = a.order[slot];
-    Packed<T> p;
-    for (int k = 0; k < 3; ++k)
+    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
     {
-        p.x[k] = a.positions[3 * old + k];
-        p.v[k] = a.velocities[3 * old + k];
-        p.f[k] = a.forces[3 * old + k];
-        p.image[k] = a.images[3 * old + k];
+        const std::int64_t old = a.order[slot];
+        Packed<T> p;
+        for (int k = 0; k < 3; ++k)
+        {
+            p.x[k] = a.positions[3 * old + k];
+            p.v[k] = a.velocities[3 * old + k];
+            p.f[k] = a.forces[3 * old + k];
+            p.image[k] = a.images[3 * old + k];
+        }
+        p.mass = a.masses[old];
+        p.inverse_mass = a.inverse_masses[old];
+        p.type = a.types ? a.types[old] : 0;
+        p.id = a.ids[old];
+        reinterpret_cast<Packed<T>*>(a.scratch)[slot] = p;
     }
-    p.mass = a.masses[old];
-    p.inverse_mass = a.inverse_masses[old];
-    p.type = a.types ? a.types[old] : 0;
-    p.id = a.ids[old];
-    reinterpret_cast<Packed<T>*>(a.scratch)[slot] = p;
 }
 
 template <typename T>
@@ -447,26 +513,28 @@ __global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int fo
 {
     using V4 = typename Vec4<T>::type;
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= a.num_atoms) return;
-    const Packed<T> p = reinterpret_cast<const Packed<T>*>(a.scratch)[slot];
-    for (int k = 0; k < 3; ++k)
+    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
     {
-        a.positions[3 * slot + k] = p.x[k];
-        a.velocities[3 * slot + k] = p.v[k];
-        a.forces[3 * slot + k] = p.f[k];
-        a.images[3 * slot + k] = p.image[k];
+        const Packed<T> p = reinterpret_cast<const Packed<T>*>(a.scratch)[slot];
+        for (int k = 0; k < 3; ++k)
+        {
+            a.positions[3 * slot + k] = p.x[k];
+            a.velocities[3 * slot + k] = p.v[k];
+            a.forces[3 * slot + k] = p.f[k];
+            a.images[3 * slot + k] = p.image[k];
+        }
+        a.masses[slot] = p.mass;
+        a.inverse_masses[slot] = p.inverse_mass;
+        if (a.types) a.types[slot] = p.type;
+        a.ids[slot] = p.id;
+        V4 p4;
+        p4.x = p.x[0];
+        p4.y = p.x[1];
+        p4.z = p.x[2];
+        p4.w = static_cast<T>(p.type);
+        reinterpret_cast<V4*>(a.positions4)[slot] = p4;
     }
-    a.masses[slot] = p.mass;
-    a.inverse_masses[slot] = p.inverse_mass;
-    if (a.types) a.types[slot] = p.type;
-    a.ids[slot] = p.id;
-    V4 p4;
-    p4.x = p.x[0];
-    p4.y = p.x[1];
-    p4.z = p.x[2];
-    p4.w = T(0);
-    reinterpret_cast<V4*>(a.positions4)[slot] = p4;
 }
 
 // Distance from coordinate c to the slab [lo, lo + w): 0 inside.
@@ -489,134 +557,138 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
 #pragma clang fp contract(off)
     using V4 = typename Vec4<T>::type;
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t center =
-        static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
-    if (center >= a.num_atoms) return;
-    const int lane = lane_id();
-    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
-    const V4 c = pos[center];
-    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
-    const index_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const index_t hx = sub_bin(c.x, a.wx, nx), hy = sub_bin(c.y, a.wy, ny),
-                  hz = sub_bin(c.z, a.wz, nz);
-    const bool full_x = nx < kStencil, full_y = ny < kStencil, full_z = nz < kStencil;
-    const int span_x = full_x ? nx : kStencil;
-    const int span_y = full_y ? ny : kStencil;
-    const int span_z = full_z ? nz : kStencil;
-    // this lane's range: stencil row lane / 2, part lane % 2 of its run of x sub-bins
-    index_t lo = 0, length = 0;
-    const int row = lane / 2, part = lane % 2;
-    if (row < span_y * span_z)
+    // one wave per atom, the waves of the grid striding over the atoms
+    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (kBlock / kWave);
+    for (std::int64_t center =
+             static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+         center < a.num_atoms; center += waves)
     {
-        const int oy = row % span_y, oz = row / span_y;
-        const index_t by = full_y ? oy : hy + oy - 2;  // unwrapped, for the distance
-        const index_t bz = full_z ? oz : hz + oz - 2;
-        const T dy = full_y ? T(0) : slab_distance(c.y, by * a.wy, a.wy);
-        const T dz = full_z ? T(0) : slab_distance(c.z, bz * a.wz, a.wz);
-        const T dyz = dy * dy + dz * dz;
-        if (dyz < cull_squared)
+        const int lane = lane_id();
+        const V4* pos = reinterpret_cast<const V4*>(a.positions4);
+        const V4 c = pos[center];
+        const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
+        const index_t nx = a.nx, ny = a.ny, nz = a.nz;
+        const index_t hx = sub_bin(c.x, a.wx, nx), hy = sub_bin(c.y, a.wy, ny),
+                      hz = sub_bin(c.z, a.wz, nz);
+        const bool full_x = nx < kStencil, full_y = ny < kStencil, full_z = nz < kStencil;
+        const int span_x = full_x ? nx : kStencil;
+        const int span_y = full_y ? ny : kStencil;
+        const int span_z = full_z ? nz : kStencil;
+        // this lane's range: stencil row lane / 2, part lane % 2 of its run of x sub-bins
+        index_t lo = 0, length = 0;
+        const int row = lane / 2, part = lane % 2;
+        if (row < span_y * span_z)
         {
-            int first = span_x, last = -1;
-            for (int ox = 0; ox < span_x; ++ox)
+            const int oy = row % span_y, oz = row / span_y;
+            const index_t by = full_y ? oy : hy + oy - 2;  // unwrapped, for the distance
+            const index_t bz = full_z ? oz : hz + oz - 2;
+            const T dy = full_y ? T(0) : slab_distance(c.y, by * a.wy, a.wy);
+            const T dz = full_z ? T(0) : slab_distance(c.z, bz * a.wz, a.wz);
+            const T dyz = dy * dy + dz * dz;
+            if (dyz < cull_squared)
             {
-                const index_t bx = full_x ? ox : hx + ox - 2;
-                const T dx = full_x ? T(0) : slab_distance(c.x, bx * a.wx, a.wx);
-                if (dx * dx + dyz < cull_squared)
+                int first = span_x, last = -1;
+                for (int ox = 0; ox < span_x; ++ox)
                 {
-                    first = ox < first ? ox : first;
-                    last = ox;
+                    const index_t bx = full_x ? ox : hx + ox - 2;
+                    const T dx = full_x ? T(0) : slab_distance(c.x, bx * a.wx, a.wx);
+                    if (dx * dx + dyz < cull_squared)
+                    {
+                        first = ox < first ? ox : first;
+                        last = ox;
+                    }
                 }
-            }
-            if (last >= first)
-            {
-                const index_t wy = by < 0 ? by + ny : (by >= ny ? by - ny : by);
-                const index_t wz = bz < 0 ? bz + nz : (bz >= nz ? bz - nz : bz);
-                const std::int64_t base =
-                    static_cast<std::int64_t>(nx) * (wy + static_cast<std::int64_t>(ny) * wz);
-                const index_t x0 = full_x ? first : hx + first - 2;
-                const index_t x1 = full_x ? last : hx + last - 2;
-                // a run that wraps around the x faces splits into two contiguous runs
-                index_t run_lo = 0, run_hi = -1;
-                if (x0 < 0)
+                if (last >= first)
                 {
-                    run_lo = part == 0 ? x0 + nx : 0;
-                    run_hi = part == 0 ? nx - 1 : x1;
-                }
-                else if (x1 >= nx)
-                {
-                    run_lo = part == 0 ? x0 : 0;
-                    run_hi = part == 0 ? nx - 1 : x1 - nx;
-                }
-                else if (part == 0)
-                {
-                    run_lo = x0;
-                    run_hi = x1;
-                }
-                if (run_hi >= run_lo)
-                {
-                    lo = a.bin_offsets[base + run_lo];
-                    length = a.bin_offsets[base + run_hi + 1] - lo;
+                    const index_t wy = by < 0 ? by + ny : (by >= ny ? by - ny : by);
+                    const index_t wz = bz < 0 ? bz + nz : (bz >= nz ? bz - nz : bz);
+                    const std::int64_t base =
+                        static_cast<std::int64_t>(nx) * (wy + static_cast<std::int64_t>(ny) * wz);
+                    const index_t x0 = full_x ? first : hx + first - 2;
+                    const index_t x1 = full_x ? last : hx + last - 2;
+                    // a run that wraps around the x faces splits into two contiguous runs
+                    index_t run_lo = 0, run_hi = -1;
+                    if (x0 < 0)
+                    {
+                        run_lo = part == 0 ? x0 + nx : 0;
+                        run_hi = part == 0 ? nx - 1 : x1;
+                    }
+                    else if (x1 >= nx)
+                    {
+                        run_lo = part == 0 ? x0 : 0;
+                        run_hi = part == 0 ? nx - 1 : x1 - nx;
+                    }
+                    else if (part == 0)
+                    {
+                        run_lo = x0;
+                        run_hi = x1;
+                    }
+                    if (run_hi >= run_lo)
+                    {
+                        lo = a.bin_offsets[base + run_lo];
+                        length = a.bin_offsets[base + run_hi + 1] - lo;
+                    }
                 }
             }
         }
-    }
-    const index_t inclusive = wave_inclusive_sum(length);
-    const index_t start = inclusive - length;
-    const index_t total = __shfl(inclusive, kWave - 1, kWave);
-    // near a face (or with a short axis) the images come from the minimum image per candidate
-    const bool wraps = full_x || full_y || full_z || hx < 2 || hx + 2 >= nx || hy < 2 ||
-                       hy + 2 >= ny || hz < 2 || hz + 2 >= nz;
-    index_t count = 0;
-    index_t* row_out = a.index + center * a.stride;
-    for (index_t chunk = 0; chunk < total; chunk += kWave)
-    {
-        const index_t t = chunk + lane;
-        // the range holding t: the last lane whose start is <= t
-        int r = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
+        const index_t inclusive = wave_inclusive_sum(length);
+        const index_t start = inclusive - length;
+        const index_t total = __shfl(inclusive, kWave - 1, kWave);
+        // near a face (or with a short axis) the images come from the minimum image per candidate
+        const bool wraps = full_x || full_y || full_z || hx < 2 || hx + 2 >= nx || hy < 2 ||
+                           hy + 2 >= ny || hz < 2 || hz + 2 >= nz;
+        index_t count = 0;
+        index_t* row_out = a.index + center * a.stride;
+        for (index_t chunk = 0; chunk < total; chunk += kWave)
         {
-            const index_t s = __shfl(start, r + step, kWave);
-            if (s <= t) r += step;
-        }
-        const index_t slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
-        bool hit = false;
-        if (t < total && slot != center)
-        {
-            const V4 q = pos[slot];
-            T dx, dy, dz;
-            if (wraps)
+            const index_t t = chunk + lane;
+            // the range holding t: the last lane whose start is <= t
+            int r = 0;
+    #pragma unroll
+            for (int step = 32; step > 0; step >>= 1)
             {
-                dx = minimum_image(c.x, q.x, lx);
-                dy = minimum_image(c.y, q.y, ly);
-                dz = minimum_image(c.z, q.z, lz);
+                const index_t s = __shfl(start, r + step, kWave);
+                if (s <= t) r += step;
             }
-            else
+            const index_t slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
+            bool hit = false;
+            if (t < total && slot != center)
             {
-                dx = c.x - q.x;
-                dy = c.y - q.y;
-                dz = c.z - q.z;
+                const V4 q = pos[slot];
+                T dx, dy, dz;
+                if (wraps)
+                {
+                    dx = minimum_image(c.x, q.x, lx);
+                    dy = minimum_image(c.y, q.y, ly);
+                    dz = minimum_image(c.z, q.z, lz);
+                }
+                else
+                {
+                    dx = c.x - q.x;
+                    dy = c.y - q.y;
+                    dz = c.z - q.z;
+                }
+                const T xx = dx * dx;
+                const T yy = dy * dy;
+                const T zz = dz * dz;
+                hit = (xx + yy) + zz < radius_squared;
             }
-            const T xx = dx * dx;
-            const T yy = dy * dy;
-            const T zz = dz * dz;
-            hit = (xx + yy) + zz < radius_squared;
+            const std::uint64_t mask = ballot(hit);
+            if (hit && a.stride > 0)
+            {
+                const index_t position = count + bits_below(mask);
+                if (position < a.stride) row_out[position] = slot;
+            }
+            count += static_cast<index_t>(__popcll(mask));
         }
-        const std::uint64_t mask = ballot(hit);
-        if (hit && a.stride > 0)
+        if (lane == 0)
         {
-            const index_t position = count + bits_below(mask);
-            if (position < a.stride) row_out[position] = slot;
-        }
-        count += static_cast<index_t>(__popcll(mask));
-    }
-    if (lane == 0)
-    {
-        a.counts[center] = count;
-        if (a.stride > 0 && count > a.stride)
-        {
-            atomicOr(a.status, kStatusFullOverflow);
-            atomicMax(a.status + 1, count);
+            a.counts[center] = count;
+            if (a.stride > 0 && count > a.stride)
+            {
+                atomicOr(a.status, kStatusFullOverflow);
+                atomicMax(a.status + 1, count);
+            }
         }
     }
 }
@@ -625,22 +697,16 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) md_commit_kernel(StepArgs<T> a, int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= 3LL * a.num_atoms) return;
-    a.reference_positions[i] = a.positions[i];
-    if (i < 3) a.reference_cell[i] = a.cell[i];
-    if (i == 0 && a.rebuild_count) ++a.rebuild_count[0];
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < 3LL * a.num_atoms; i += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    {
+        a.reference_positions[i] = a.positions[i];
+        if (i < 3) a.reference_cell[i] = a.cell[i];
+        if (i == 0 && a.rebuild_count) ++a.rebuild_count[0];
+    }
 }
 
 // ---- forces ---------------------------------------------------------------------------------
-
-// Blocks are dealt to the 8 XCDs round-robin; renumbering them puts consecutive logical blocks
-// (neighboring rows, sharing neighbor positions) on one XCD and its L2. grid is a multiple of 8.
-__device__ __forceinline__ unsigned xcd_block(unsigned block, unsigned grid)
-{
-    const unsigned per = grid / 8;
-    return (block % 8) * per + block / 8;
-}
 
 inline unsigned xcd_grid(std::int64_t blocks)
 {
@@ -655,109 +721,170 @@ __device__ __forceinline__ V group_sum(V value)
     return value;
 }
 
-// G lanes per row: each reads G consecutive entries of its row at a time. Wave-uniform choice
-// of images: a wave whose atoms all sit farther than the cutoff (plus a margin) from every face
-// takes x_i - x_j directly, which is the minimum image for every pair within the cutoff; any
-// other image it may pick is longer than L/2 > rc, so the mask drops it exactly as the
-// reference does. Other waves use the reference's displacement (space.py:11-19) per pair.
+// One group's walk over its row: each lane takes 4 consecutive entries (one 16-byte load) per
+// pass, the group 4G entries, and the lane's four position gathers are issued together; the
+// next pass's entries are loaded before this pass's pairs are evaluated. Entries at or past
+// `count` are masked, their gathers read the row's own atom. The row start is 16-byte aligned
+// and `width` <= stride, a multiple of 8, so no load leaves the row.
+template <int G, bool kFast, bool kVirial, typename T, typename V4, typename Ev>
+__device__ __forceinline__ void walk_row(const Ev& ev, const V4& c, const V4* __restrict__ pos,
+                                         const index_t* __restrict__ entries, index_t count,
+                                         index_t width, index_t self, T lx, T ly, T lz,
+                                         T cutoff_squared, T& fx, T& fy, T& fz, T& energy, T* w)
+{
+    index_t base = 4 * (lane_id() % G);
+    // loads are unconditional, their offsets clamped into the row (stride >= 8 entries)
+    int4 quad = *reinterpret_cast<const int4*>(entries + (base < width ? base : 0));
+    while (base < width)
+    {
+        const index_t next = base + 4 * G;
+        const int4 ahead = *reinterpret_cast<const int4*>(entries + (next < width ? next : base));
+        const index_t js[4] = {quad.x, quad.y, quad.z, quad.w};
+        V4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = pos[base + u < count ? js[u] : self];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            T dx, dy, dz;
+            if (kFast)
+            {
+                dx = c.x - q[u].x;
+                dy = c.y - q[u].y;
+                dz = c.z - q[u].z;
+            }
+            else
+            {
+                dx = displacement(c.x, q[u].x, lx);
+                dy = displacement(c.y, q[u].y, ly);
+                dz = displacement(c.z, q[u].z, lz);
+            }
+            const T r2 = exact_squared(dx, dy, dz);
+            const bool inside = base + u < count && r2 < cutoff_squared;
+            T pu, fs;
+            ev.eval(c.w, q[u].w, r2, pu, fs);
+            // a select, not a product: a masked pair's fs may be inf or NaN
+            pu = inside ? pu : T(0);
+            fs = inside ? fs : T(0);
+            energy += pu;
+            fx += fs * dx;
+            fy += fs * dy;
+            fz += fs * dz;
+            if (kVirial)
+            {
+                w[0] += dx * (fs * dx);
+                w[1] += dy * (fs * dy);
+                w[2] += dz * (fs * dz);
+                w[3] += dx * (fs * dy);
+                w[4] += dx * (fs * dz);
+                w[5] += dy * (fs * dz);
+            }
+        }
+        quad = ahead;
+        base = next;
+    }
+}
+
+// Waves per SIMD the force kernel is compiled for: each potential's `waves`, the most its fp32
+// kernels without the virial reach without spilling (one-type LJ 8, typed LJ 7, table 5). The
+// virial's six more sums, and fp64, need more registers: those are left to the compiler.
+template <typename T, class Pot, bool kVirial>
+constexpr int kForceWaves = kVirial ? 1 : Pot::template waves<T>;
+
+// The full list's rows in tiles of kBlock / G rows, G lanes per row. The grid is persistent: the
+// blocks of one XCD (blockIdx % 8; blocks are dealt to the 8 XCDs round-robin) walk a contiguous
+// eighth of the tiles, so neighboring rows, which gather the same positions, share that XCD's
+// L2. Each block sums its E, W and K in a fixed order into its partial slot, so every run gives
+// the same bits.
+//
+// Images are chosen once per wave: a wave whose atoms all sit farther than the cutoff (plus a
+// margin) from every face takes x_i - x_j directly, which is the minimum image for every pair
+// within the cutoff; any other image it may pick is longer than L/2 > rc, so the mask drops it
+// exactly as the reference does. Other waves use the reference's displacement (space.py:11-19)
+// per pair.
 template <typename T, class Pot, int G, bool kVirial, bool kKick>
-__global__ void __launch_bounds__(kBlock) force_kernel(StepArgs<T> a, Pot pot, T cutoff,
+__global__ void __launch_bounds__(kBlock, (kForceWaves<T, Pot, kVirial>)) force_kernel(StepArgs<T> a, Pot pot, T cutoff,
                                                        T cutoff_squared, T half_timestep,
                                                        double* __restrict__ partials,
-                                                       std::int64_t real_blocks)
+                                                       index_t tiles)
 {
     using V4 = typename Vec4<T>::type;
     __shared__ double shared[16];
-    constexpr int kRowsPerBlock = kBlock / G;
-    const unsigned block = xcd_block(blockIdx.x, gridDim.x);
-    const int lane = lane_id();
-    const int sub = lane % G;
-    const std::int64_t row = static_cast<std::int64_t>(block) * kRowsPerBlock + threadIdx.x / G;
-    const bool live = block < real_blocks && row < a.num_atoms;
-    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
-    V4 c{};
-    index_t count = 0;
-    if (live)
-    {
-        c = pos[row];
-        // a row that overflowed holds only `stride` entries (the status bit reports it)
-        const index_t listed = a.counts[row];
-        count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
-    }
+    constexpr int kRows = kBlock / G;
+    const auto ev = pot.bind();
+    const int sub = lane_id() % G;
+    const V4* __restrict__ pos = reinterpret_cast<const V4*>(a.positions4);
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const T low = cutoff + T(1e-3);
-    const bool interior = !live || (c.x > low && c.x < lx - low && c.y > low && c.y < ly - low &&
-                                    c.z > low && c.z < lz - low);
-    const bool fast = ballot(!interior) == 0ull;
-    const index_t width = wave_max(count);
-    const index_t* __restrict__ entries = a.index + (live ? row : 0) * a.stride;
-    T fx = 0, fy = 0, fz = 0, energy = 0;
-    T w[6] = {0, 0, 0, 0, 0, 0};
-    for (index_t k = sub; k < width; k += G)
+    const unsigned xcd = blockIdx.x % 8;
+    const index_t first = static_cast<index_t>(static_cast<std::int64_t>(tiles) * xcd / 8);
+    const index_t last = static_cast<index_t>(static_cast<std::int64_t>(tiles) * (xcd + 1) / 8);
+    const index_t step = static_cast<index_t>(gridDim.x / 8);
+    double energy_sum = 0.0, kinetic = 0.0;
+    double w_sum[6] = {0, 0, 0, 0, 0, 0};
+    for (index_t tile = first + static_cast<index_t>(blockIdx.x / 8); tile < last; tile += step)
     {
-        if (k >= count) continue;
-        const index_t j = entries[k];
-        const V4 q = pos[j];
-        T dx, dy, dz;
+        const index_t row = tile * kRows + static_cast<index_t>(threadIdx.x) / G;
+        const bool live = row < a.num_atoms;
+        const index_t self = live ? row : 0;
+        const V4 c = pos[self];
+        // a row that overflowed holds only `stride` entries (the status bit reports it)
+        const index_t listed = live ? a.counts[row] : 0;
+        const index_t count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
+        const bool interior = !live || (c.x > low && c.x < lx - low && c.y > low &&
+                                        c.y < ly - low && c.z > low && c.z < lz - low);
+        const bool fast = ballot(!interior) == 0ull;
+        const index_t width = wave_max(count);
+        const index_t* entries = a.index + static_cast<std::int64_t>(self) * a.stride;
+        T fx = 0, fy = 0, fz = 0, energy = 0;
+        T w[6] = {0, 0, 0, 0, 0, 0};
         if (fast)
-        {
-            dx = c.x - q.x;
-            dy = c.y - q.y;
-            dz = c.z - q.z;
-        }
+            walk_row<G, true, kVirial>(ev, c, pos, entries, count, width, self, lx, ly, lz,
+                                       cutoff_squared, fx, fy, fz, energy, w);
         else
-        {
-            dx = displacement(c.x, q.x, lx);
-            dy = displacement(c.y, q.y, ly);
-            dz = displacement(c.z, q.z, lz);
-        }
-        const T r2 = exact_squared(dx, dy, dz);
-        if (!(r2 < cutoff_squared)) continue;
-        T u, fs;
-        pot.eval(pot.row(row, j), r2, u, fs);
-        energy += u;
-        fx += fs * dx;
-        fy += fs * dy;
-        fz += fs * dz;
+            walk_row<G, false, kVirial>(ev, c, pos, entries, count, width, self, lx, ly, lz,
+                                        cutoff_squared, fx, fy, fz, energy, w);
+        fx = group_sum<G>(fx);
+        fy = group_sum<G>(fy);
+        fz = group_sum<G>(fz);
+        energy_sum += static_cast<double>(energy);
         if (kVirial)
+            for (int k = 0; k < 6; ++k) w_sum[k] += static_cast<double>(w[k]);
+        if (live && sub == 0)
         {
-            w[0] += dx * (fs * dx);
-            w[1] += dy * (fs * dy);
-            w[2] += dz * (fs * dz);
-            w[3] += dx * (fs * dy);
-            w[4] += dx * (fs * dz);
-            w[5] += dy * (fs * dz);
-        }
-    }
-    fx = group_sum<G>(fx);
-    fy = group_sum<G>(fy);
-    fz = group_sum<G>(fz);
-    double kinetic = 0.0;
-    if (live && sub == 0)
-    {
-        T* f = a.forces + 3 * row;
-        f[0] = fx;
-        f[1] = fy;
-        f[2] = fz;
-        if (kKick)
-        {
+            T* f = a.forces + 3LL * row;
+            f[0] = fx;
+            f[1] = fy;
+            f[2] = fz;
+            if (kKick)
+            {
 #pragma clang fp contract(off)
-            T* v = a.velocities + 3 * row;
-            const T inverse_mass = a.inverse_masses[row];
-            const T vx = v[0] + half_timestep * fx * inverse_mass;
-            const T vy = v[1] + half_timestep * fy * inverse_mass;
-            const T vz = v[2] + half_timestep * fz * inverse_mass;
-            v[0] = vx;
-            v[1] = vy;
-            v[2] = vz;
-            kinetic = 0.5 * static_cast<double>(a.masses[row]) *
-                      (static_cast<double>(vx) * vx + static_cast<double>(vy) * vy +
-                       static_cast<double>(vz) * vz);
+                T* v = a.velocities + 3LL * row;
+                const T inverse_mass = a.inverse_masses[row];
+                const T vx = v[0] + half_timestep * fx * inverse_mass;
+                const T vy = v[1] + half_timestep * fy * inverse_mass;
+                const T vz = v[2] + half_timestep * fz * inverse_mass;
+                v[0] = vx;
+                v[1] = vy;
+                v[2] = vz;
+                kinetic += 0.5 * static_cast<double>(a.masses[row]) *
+                           (static_cast<double>(vx) * vx + static_cast<double>(vy) * vy +
+                            static_cast<double>(vz) * vz);
+            }
         }
     }
-    store_partials<kVirial>(static_cast<double>(energy), w, shared, partials);
-    const double k_sum = block_sum(kinetic, shared);
-    if (threadIdx.x == 0) partials[8LL * blockIdx.x + 7] = k_sum;
+    double* out = partials + 8LL * blockIdx.x;
+    const double e = block_sum(energy_sum, shared);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (kVirial)
+        for (int k = 0; k < 6; ++k) v[k] = block_sum(w_sum[k], shared);
+    const double k_sum = kKick ? block_sum(kinetic, shared) : 0.0;
+    if (threadIdx.x == 0)
+    {
+        out[0] = e;
+        for (int k = 0; k < 6; ++k) out[1 + k] = v[k];
+        out[7] = k_sum;
+    }
 }
 
 // E, W and K from the partials (pairs counted twice, so E and W halve); then the chain's
@@ -883,125 +1010,172 @@ StepArgs<T> step_args(const ndn_md_system& s, const ndn_md_settings* settings,
     return a;
 }
 
+// Grid of a rebuild kernel gated on the device flag. On the many steps without a rebuild every
+// block exits at once, so the grid is capped (its blocks stride over the items) to keep those
+// idle launches short; 4096 blocks of 256 threads still fill the chip when it does rebuild.
+constexpr std::int64_t kGatedBlocks = 4096;
+
+inline unsigned gated_grid(std::int64_t items, int per_block = kBlock)
+{
+    return static_cast<unsigned>(
+        std::max<std::int64_t>(1, std::min<std::int64_t>(blocks_for(items, per_block), kGatedBlocks)));
+}
+
 template <typename T>
 int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream_t stream)
 {
     const std::int64_t n = a.num_atoms;
-    md_assign_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    md_assign_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_assign");
     const index_t* gate = forced ? nullptr : a.rebuild;
     const int scanned = exclusive_scan(a.bin_counts, a.bin_offsets, a.bins + 1, a.scan_temp, gate,
                                        a.status, stream);
     if (scanned != NDN_OK) return scanned;
-    md_scatter_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    md_scatter_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_scatter");
-    md_order_kernel<T><<<blocks_for(a.bins), kBlock, 0, stream>>>(a, forced);
+    md_order_kernel<T><<<gated_grid(a.bins), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_order");
-    md_permute_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    md_permute_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_permute");
-    md_unpack_kernel<T><<<blocks_for(n), kBlock, 0, stream>>>(a, forced);
+    md_unpack_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_unpack");
     const T r = static_cast<T>(radius);
     const T r2 = r * r;
     // culling keeps every sub-bin a pair within R could sit in, with room for rounding
     const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
-    md_fill_kernel<T><<<blocks_for(n, kBlock / kWave), kBlock, 0, stream>>>(a, r2, cull * cull,
+    md_fill_kernel<T><<<gated_grid(n, kBlock / kWave), kBlock, 0, stream>>>(a, r2, cull * cull,
                                                                             forced);
     NDN_LAUNCHED("md_fill");
-    md_commit_kernel<T><<<blocks_for(3 * n), kBlock, 0, stream>>>(a, forced);
+    md_commit_kernel<T><<<gated_grid(3 * n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_commit");
     return NDN_OK;
 }
 
-std::int64_t force_blocks(std::int64_t num_atoms, int group)
+// Tiles of kBlock / G rows.
+std::int64_t force_tiles(std::int64_t num_atoms, int group)
 {
-    const std::int64_t rows_per_block = kBlock / group;
-    return (num_atoms + rows_per_block - 1) / rows_per_block;
+    const std::int64_t rows_per_tile = kBlock / group;
+    return (num_atoms + rows_per_tile - 1) / rows_per_tile;
+}
+
+// The persistent grid: as many blocks as are resident on the device at once, a multiple of 8
+// (one share per XCD), and no more than the tiles need (so never more than xcd_grid(tiles)).
+template <typename Kernel>
+unsigned persistent_grid(Kernel kernel, std::int64_t tiles)
+{
+    int device = 0, units = 0, per_unit = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, kernel, kBlock, 0) != hipSuccess)
+        return xcd_grid(tiles);
+    const std::int64_t resident =
+        static_cast<std::int64_t>(std::max(units, 1)) * std::max(per_unit, 1);
+    return xcd_grid(std::min(resident, tiles));
+}
+
+template <typename T, class Pot, int G, bool kVirial, bool kKick>
+unsigned launch_force_kernel(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
+                             T half_timestep, double* partials, hipStream_t stream)
+{
+    const std::int64_t tiles = force_tiles(a.num_atoms, G);
+    auto kernel = force_kernel<T, Pot, G, kVirial, kKick>;
+    const unsigned grid = persistent_grid(kernel, tiles);
+    kernel<<<grid, kBlock, 0, stream>>>(a, pot, cutoff, cutoff_squared, half_timestep, partials,
+                                        static_cast<index_t>(tiles));
+    return grid;
 }
 
 template <typename T, class Pot, int G>
-void launch_force_g(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
-                    T half_timestep, bool virial, bool kick, double* partials, hipStream_t stream)
+unsigned launch_force_g(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
+                        T half_timestep, bool virial, bool kick, double* partials,
+                        hipStream_t stream)
 {
-    const std::int64_t real = force_blocks(a.num_atoms, G);
-    const unsigned grid = xcd_grid(real);
     if (virial && kick)
-        force_kernel<T, Pot, G, true, true><<<grid, kBlock, 0, stream>>>(
-            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
-    else if (virial)
-        force_kernel<T, Pot, G, true, false><<<grid, kBlock, 0, stream>>>(
-            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
-    else if (kick)
-        force_kernel<T, Pot, G, false, true><<<grid, kBlock, 0, stream>>>(
-            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
-    else
-        force_kernel<T, Pot, G, false, false><<<grid, kBlock, 0, stream>>>(
-            a, pot, cutoff, cutoff_squared, half_timestep, partials, real);
+        return launch_force_kernel<T, Pot, G, true, true>(a, pot, cutoff, cutoff_squared,
+                                                          half_timestep, partials, stream);
+    if (virial)
+        return launch_force_kernel<T, Pot, G, true, false>(a, pot, cutoff, cutoff_squared,
+                                                           half_timestep, partials, stream);
+    if (kick)
+        return launch_force_kernel<T, Pot, G, false, true>(a, pot, cutoff, cutoff_squared,
+                                                           half_timestep, partials, stream);
+    return launch_force_kernel<T, Pot, G, false, false>(a, pot, cutoff, cutoff_squared,
+                                                        half_timestep, partials, stream);
 }
 
 template <typename T, class Pot>
-void launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff, T cutoff_squared,
-                  T half_timestep, bool virial, bool kick, double* partials, hipStream_t stream)
+unsigned launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff,
+                      T cutoff_squared, T half_timestep, bool virial, bool kick,
+                      double* partials, hipStream_t stream)
 {
     switch (group)
     {
-        case 1:
-            launch_force_g<T, Pot, 1>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
-                                      partials, stream);
-            break;
+        case 2:
+            return launch_force_g<T, Pot, 2>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                             virial, kick, partials, stream);
         case 4:
-            launch_force_g<T, Pot, 4>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
-                                      partials, stream);
-            break;
+            return launch_force_g<T, Pot, 4>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                             virial, kick, partials, stream);
         case 8:
-            launch_force_g<T, Pot, 8>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
-                                      partials, stream);
-            break;
-        case 32:
-            launch_force_g<T, Pot, 32>(a, pot, cutoff, cutoff_squared, half_timestep, virial,
-                                       kick, partials, stream);
-            break;
+            return launch_force_g<T, Pot, 8>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                             virial, kick, partials, stream);
         default:
-            launch_force_g<T, Pot, 16>(a, pot, cutoff, cutoff_squared, half_timestep, virial,
-                                       kick, partials, stream);
+            return launch_force_g<T, Pot, 16>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                              virial, kick, partials, stream);
     }
 }
 
-constexpr int kMaxGroup = 32;
+// The widest lane group. A tile holds kBlock / G rows, so the widest group has the most tiles;
+// the persistent grid never exceeds xcd_grid(tiles), so xcd_grid(force_tiles(N, kMaxGroup))
+// partial slots suffice for every schedule.
+constexpr int kMaxGroup = 16;
 
 int valid_group(int group)
 {
-    return group == 1 || group == 4 || group == 8 || group == 16 || group == 32;
+    return group == 2 || group == 4 || group == 8 || group == 16;
 }
 
 template <typename T>
 int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coefficients, int group,
                   bool virial, bool kick, double* partials, std::int64_t partial_slots,
-                  hipStream_t stream)
+                  hipStream_t stream, int* grid_out)
 {
-    NDN_REQUIRE(valid_group(group), "group must be 1, 4, 8, 16 or 32");
-    NDN_REQUIRE(static_cast<std::int64_t>(xcd_grid(force_blocks(a.num_atoms, group))) <= partial_slots,
-                "the force launch has more blocks than partial slots");
+    NDN_REQUIRE(valid_group(group), "group must be 2, 4, 8 or 16");
+    NDN_REQUIRE(static_cast<std::int64_t>(xcd_grid(force_tiles(a.num_atoms, group))) <=
+                    partial_slots,
+                "the force launch may have more blocks than partial slots");
+    NDN_REQUIRE(a.stride >= 8 && a.stride % 4 == 0,
+                "the list stride must be a multiple of 4 and at least 8");
     const T cutoff = static_cast<T>(pot->cutoff);
     const T cutoff_squared = static_cast<T>(pot->cutoff * pot->cutoff);
     const T half_timestep = static_cast<T>(0.5 * a.timestep);
     const int rows = pot->num_types * (pot->num_types + 1) / 2;
-    if (pot->kind == NDN_LJ)
+    unsigned grid;
+    if (pot->kind == NDN_LJ && pot->num_types == 1)
     {
-        const FastLJ<T> lj{static_cast<const LJRow<T>*>(coefficients), a.types, pot->num_types};
-        launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                        partials, stream);
+        const OneTypeLJ<T> lj{static_cast<const LJRow<T>*>(coefficients)};
+        grid = launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                               partials, stream);
+    }
+    else if (pot->kind == NDN_LJ)
+    {
+        const TypedLJ<T> lj{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
+        grid = launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                               partials, stream);
     }
     else
     {
         T* padded = static_cast<T*>(coefficients);
-        const Table<T> table{padded, static_cast<const T*>(pot->start),
-                             padded + static_cast<std::int64_t>(rows) * (pot->knots + 2), a.types,
-                             pot->num_types, static_cast<int>(pot->knots)};
-        launch_force<T>(a, table, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                        partials, stream);
+        const TypedTable<T> table{Table<T>{
+            padded, static_cast<const T*>(pot->start),
+            padded + static_cast<std::int64_t>(rows) * (pot->knots + 2), nullptr,
+            pot->num_types, static_cast<int>(pot->knots)}};
+        grid = launch_force<T>(a, table, group, cutoff, cutoff_squared, half_timestep, virial,
+                               kick, partials, stream);
     }
     NDN_LAUNCHED("md_force");
+    *grid_out = static_cast<int>(grid);
     return NDN_OK;
 }
 
@@ -1018,10 +1192,10 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     NDN_LAUNCHED("md_first_half");
     const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream);
     if (rebuilt != NDN_OK) return rebuilt;
+    int blocks = 0;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, settings->virial != 0,
-                                        true, s.partials, s.partial_slots, stream);
+                                        true, s.partials, s.partial_slots, stream, &blocks);
     if (forces != NDN_OK) return forces;
-    const int blocks = static_cast<int>(xcd_grid(force_blocks(s.num_atoms, m.group)));
     finish_step_kernel<T><<<1, kBlock, 0, stream>>>(a, s.partials, blocks, 1);
     NDN_LAUNCHED("md_finish");
     return NDN_OK;
@@ -1036,10 +1210,10 @@ int md_evaluate_impl(const ndn_md_system& s, const ndn_potential* pot, const ndn
     PotentialView<T> view;
     const int prepared = prepare_potential<T>(pot, s.coefficients, stream, &view);
     if (prepared != NDN_OK) return prepared;
+    int blocks = 0;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, virial != 0, kick != 0,
-                                        s.partials, s.partial_slots, stream);
+                                        s.partials, s.partial_slots, stream, &blocks);
     if (forces != NDN_OK) return forces;
-    const int blocks = static_cast<int>(xcd_grid(force_blocks(s.num_atoms, m.group)));
     finish_step_kernel<T><<<1, kBlock, 0, stream>>>(a, s.partials, blocks, 0);
     NDN_LAUNCHED("md_evaluate_finish");
     return NDN_OK;
@@ -1064,9 +1238,9 @@ int check_system(const ndn_md_system* s, const ndn_md_list* m)
                 "scan_temp is too small");
     NDN_REQUIRE(m->stride == 0 || m->index != nullptr, "a list with rows needs its index");
     NDN_REQUIRE(m->stride >= 0 && m->stride * s->num_atoms <= (1LL << 40), "bad stride");
-    NDN_REQUIRE(valid_group(m->group), "group must be 1, 4, 8, 16 or 32");
+    NDN_REQUIRE(valid_group(m->group), "group must be 2, 4, 8 or 16");
     NDN_REQUIRE(s->partial_slots >=
-                    static_cast<std::int64_t>(xcd_grid(force_blocks(s->num_atoms, m->group))),
+                    static_cast<std::int64_t>(xcd_grid(force_tiles(s->num_atoms, m->group))),
                 "too few partial slots for this force schedule");
     return NDN_OK;
 }
@@ -1084,8 +1258,8 @@ int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     NDN_REQUIRE(num_atoms >= 1 && partial_slots && coefficient_bytes_out, "bad arguments");
-    // the most blocks any schedule launches: 32 lanes per row, 8 rows per block
-    *partial_slots = std::max<std::int64_t>(xcd_grid(force_blocks(num_atoms, kMaxGroup)), 8);
+    // the most blocks any schedule may launch: the widest lane group's tile count
+    *partial_slots = std::max<std::int64_t>(xcd_grid(force_tiles(num_atoms, kMaxGroup)), 8);
     *coefficient_bytes_out = coefficient_bytes(pot, dtype == NDN_F32 ? 4 : 8);
     return NDN_OK;
 }
@@ -1164,14 +1338,18 @@ int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* po
     {
         StepArgs<float> a = step_args<float>(*system, nullptr, nullptr, *md_list);
         a.timestep = timestep;
+        int grid = 0;
         return launch_forces<float>(a, pot, system->coefficients, md_list->group, virial != 0,
-                                    kick != 0, system->partials, system->partial_slots, stream);
+                                    kick != 0, system->partials, system->partial_slots, stream,
+                                    &grid);
     }
     NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     StepArgs<double> a = step_args<double>(*system, nullptr, nullptr, *md_list);
     a.timestep = timestep;
+    int grid = 0;
     return launch_forces<double>(a, pot, system->coefficients, md_list->group, virial != 0,
-                                 kick != 0, system->partials, system->partial_slots, stream);
+                                 kick != 0, system->partials, system->partial_slots, stream,
+                                 &grid);
 }
 
 int ndn_md_settle(int dtype, const ndn_md_system* system, ndn_stream_t stream)

@@ -79,7 +79,7 @@ class MDList(_native.ctypes.Structure):
     ]
 
 
-_GROUPS = (1, 4, 8, 16, 32)
+_GROUPS = (2, 4, 8, 16)
 _INVALID_CELL, _NON_FINITE, _OVERFLOW, _FULL_OVERFLOW = 1, 2, 8, 16
 
 
@@ -259,7 +259,7 @@ class Simulation:
                  cell: Tensor, *, timestep: Real, skin: Real, capacity_multiplier: Real,
                  images: Tensor | None, step: Integral, temperature: Real | None = None,
                  thermostat_timescale: Real | None = None, thermostat: ChainState | None = None,
-                 virial_every_step: bool = False):
+                 virial_every_step: bool = False, force_group: int | None = None):
         if not isinstance(model, Potential):
             raise TypeError("model must be a Potential")
         num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
@@ -360,7 +360,14 @@ class Simulation:
         self._rebuild(forced=True)
         self._neighbors._check_status()
         self._neighbors.rebuild_count.fill_(1)
-        self.group = self._tune_group()
+        if force_group is None:
+            self.group = self._tune_group()
+        else:
+            # a fixed schedule: the force sums' order, hence every bit, independent of timing
+            if force_group not in _GROUPS:
+                raise ValueError(f"force_group must be one of {_GROUPS}")
+            self._neighbors.native.group = force_group
+            self.group = force_group
         self._evaluate(virial=True)
         self.kinetic.copy_(0.5 * (self.slot_masses.double()[:, None]
                                   * self.velocities.double().square()).sum())
@@ -392,8 +399,8 @@ class Simulation:
         )
 
     def _tune_group(self, launches: int = 3) -> int:
-        """The autotuner of the force schedule: lanes per neighbor row (1 = a lane per atom,
-        up to 32), each timed with HIP events on this frame; the fastest wins. Syncs."""
+        """The autotuner of the force schedule: lanes per neighbor row (2 to 16), each timed with
+        HIP events on this frame; the fastest wins. Syncs."""
         best, best_time = 16, math.inf
         for group in _GROUPS:
             self._neighbors.native.group = group
@@ -472,6 +479,20 @@ class Simulation:
         samples = {"step": torch.tensor(due, dtype=torch.int64)}
         samples.update({name: table[:, k] for k, name in enumerate(self.fields)})
         return samples
+
+    def advance(self, steps: Integral) -> None:
+        """Queues ``steps`` graph replays on the current stream and returns at once: no host
+        sync and no status read (``check`` reads it). For schedulers that keep several
+        simulations in flight on their own streams."""
+        steps = _count("steps", steps)
+        self._require()
+        for _ in range(steps):
+            self._graph.replay()
+        self._step += steps
+
+    def check(self) -> None:
+        """Reads the neighbor status (syncs) and raises ValueError if a step failed."""
+        self._check(self._step)
 
     @property
     def fields(self) -> tuple[str, ...]:

@@ -21,23 +21,7 @@ pytestmark = pytest.mark.gpu
 
 def _systems(device, dtype, count=24, high=4096, seed=2):
     raw = workloads.ragged_systems(count, high=high, seed=seed, min_length=2.45)
-    table = workloads.lj_table()
-    out = []
-    for sysd in raw:
-        n = sysd["positions"].shape[0]
-        x = torch.tensor(sysd["positions"], dtype=dtype, device=device)
-        cell = torch.tensor(sysd["cell"], dtype=dtype, device=device)
-        types = torch.zeros(n, dtype=torch.int64, device=device)
-        if sysd["kind"] == "lj":
-            model = potential.lennard_jones(
-                atom_types=types, sigma=torch.tensor([0.34], dtype=dtype, device=device),
-                epsilon=torch.tensor([0.996], dtype=dtype, device=device), cutoff=1.0)
-        else:
-            model = potential.tabulated(
-                atom_types=types, table=torch.tensor(table[None], dtype=dtype, device=device),
-                start=torch.tensor([0.3], dtype=dtype, device=device), cutoff=1.2)
-        out.append((x, cell, model))
-    return out
+    return batch.make_systems(raw, device, dtype)
 
 
 def _reference(x, cell, model):

@@ -182,10 +182,14 @@ def test_md_full_list_pair_set(device, dtype, n, length):
 
 def test_graph_replay_equals_eager_launches(device):
     x, v, m, L = _config1(torch.float32, device)
-    a = md.nvt(_lj(512, torch.float32, device), x.to(device), v.to(device), m.to(device), L.to(device),
-               timestep=0.004, temperature=120.0, thermostat_timescale=0.1)
-    b = md.nvt(_lj(512, torch.float32, device), x.to(device), v.to(device), m.to(device), L.to(device),
-               timestep=0.004, temperature=120.0, thermostat_timescale=0.1)
+    # one fixed force schedule for both: the autotuner's timing-based choice may differ between
+    # two simulations, and a different lane group sums the forces in a different order
+    kw = dict(timestep=0.004, skin=0.1, capacity_multiplier=1.25, images=None, step=0,
+              temperature=120.0, thermostat_timescale=0.1, force_group=8)
+    a = md.Simulation(_lj(512, torch.float32, device), x.to(device), v.to(device), m.to(device),
+                      L.to(device), **kw)
+    b = md.Simulation(_lj(512, torch.float32, device), x.to(device), v.to(device), m.to(device),
+                      L.to(device), **kw)
     a.run(50)
     for _ in range(50):
         b._launch_step()
