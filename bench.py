@@ -254,17 +254,18 @@ def main() -> int:
     sim.run(args.warmup)
     torch.cuda.synchronize()
     rebuilds_before = sim._neighbors._rebuild_count
+    refreshes_before = sim._neighbors.refresh_count if sim._neighbors.inner_skin else 0
     pairs_before = sim._neighbors.num_pairs
     log(f"timing {args.steps} steps ...")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim._graph.replay()
+    sim._replay(args.steps)  # 25-step graphs, then single steps
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     sim._step += args.steps
     sim._check(sim._step)
     rebuilds = sim._neighbors._rebuild_count - rebuilds_before
+    refreshes = (sim._neighbors.refresh_count - refreshes_before) if sim._neighbors.inner_skin else 0
     pairs_after = sim._neighbors.num_pairs
     steps_per_rebuild = args.steps / rebuilds if rebuilds else float(args.steps)
     value = n * args.steps / elapsed
@@ -309,6 +310,8 @@ def main() -> int:
             "pairs_within_radius": pairs,
             "steps_per_rebuild": steps_per_rebuild,
             "force_schedule_lanes_per_row": sim.group,
+            "pruned_list_skin_nm": sim._neighbors.inner_skin,
+            "steps_per_pruned_refresh": args.steps / refreshes if refreshes else None,
             "parallelism": "single GPU",
         },
         "roofline": {

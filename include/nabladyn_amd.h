@@ -197,10 +197,12 @@ typedef struct ndn_md_list {
     int32_t* unordered;         /* [N] */
     int32_t* order;             /* [N] new slot -> previous slot */
     int32_t* counts;            /* [N] neighbors of each slot */
-    int32_t* index;             /* [N * stride] neighbor slots, row-major; NULL while stride is 0 */
+    int32_t* index;             /* [N * stride] neighbor slots, row-major, each row padded to the
+                                   stride with slot N; NULL while stride is 0 */
     int32_t* ids;               /* [N] atom id of each slot */
     int64_t* types;             /* [N] atom type of each slot, or NULL with one type */
-    void* positions4;           /* [N, 4] T */
+    void* positions4;           /* [N + 1, 4] T: x, y, z, type of each slot; row N the sentinel
+                                   (1e18, 1e18, 1e18) that pads the rows, written by each build */
     void* scratch;              /* [ndn_md_scratch_bytes] */
     uint8_t* scan_temp;
     int64_t scan_temp_bytes;
@@ -208,6 +210,14 @@ typedef struct ndn_md_list {
     int32_t* prepare_dims;      /* [3] scratch of the reuse rule */
     void* prepare_widths;       /* [3] T scratch */
     int32_t* prepare_total;     /* [1] scratch */
+    /* The pruned (inner) list: the pairs of the full list within inner_radius = cutoff + a
+     * smaller skin, refreshed from the full list after every build and whenever an atom has
+     * moved half that skin since the last refresh; the force kernel reads it instead of the full
+     * rows. inner_index NULL (inner_radius 0) reads the full rows. */
+    double inner_radius;
+    int32_t* inner_counts;      /* [N] */
+    int32_t* inner_index;       /* [N * stride], rows padded with slot N */
+    void* prune_positions;      /* [N, 3] T: the frame at the last refresh */
 } ndn_md_list;
 
 /* Scratch sizes of a system: partial slots and potential-coefficient bytes. */

@@ -7,6 +7,7 @@ stream order with the surrounding torch work and can be captured in a HIP graph.
 """
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
@@ -53,7 +54,10 @@ class NativeError(RuntimeError):
 
 
 def library_path() -> Path:
-    return _LIBRARY
+    """The library to load: ``NABLADYN_AMD_LIBRARY`` if set (another build of the same ABI),
+    else the in-tree build."""
+    override = os.environ.get("NABLADYN_AMD_LIBRARY")
+    return Path(override) if override else _LIBRARY
 
 
 def lib() -> ctypes.CDLL:
@@ -62,11 +66,12 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         with _LOCK:
             if _lib is None:
-                if not _LIBRARY.exists():
+                path = library_path()
+                if not path.exists():
                     raise RuntimeError(
-                        f"{_LIBRARY} is missing: build it with `python -m nabladyn_amd.build`"
+                        f"{path} is missing: build it with `python -m nabladyn_amd.build`"
                     )
-                handle = ctypes.CDLL(str(_LIBRARY))
+                handle = ctypes.CDLL(str(path))
                 handle.ndn_last_error.restype = ctypes.c_char_p
                 _lib = handle
     return _lib

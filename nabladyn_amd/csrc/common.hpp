@@ -50,6 +50,24 @@ __device__ __forceinline__ std::uint64_t ballot(bool predicate)
     return __ballot(predicate);
 }
 
+// The value `lane` (wave-uniform) holds: a scalar read, no LDS round trip.
+__device__ __forceinline__ int read_lane(int value, int lane)
+{
+    return __builtin_amdgcn_readlane(value, lane);
+}
+
+__device__ __forceinline__ float read_lane(float value, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(value), lane));
+}
+
+__device__ __forceinline__ double read_lane(double value, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(value), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(value), lane);
+    return __hiloint2double(hi, lo);
+}
+
 template <typename V>
 __device__ __forceinline__ V wave_inclusive_sum(V value)
 {

@@ -21,6 +21,13 @@ from ..units import BOLTZMANN
 from ._state import ChainState, State
 
 _STATUS_INTERVAL = 1000
+# Steps captured in the multi-step graph: one replay launches this many steps, so the host
+# issues one call per 25 steps (small systems' steps take less GPU time than a host replay).
+_CHUNK_STEPS = 25
+# Skin of the pruned list the force kernel reads (nm): the pairs of the full list within
+# cutoff + INNER_SKIN, refreshed whenever an atom has moved half of it. Only the force kernel's
+# work depends on it (the full list, its skin and its rebuild rule are the reference's).
+INNER_SKIN = 0.05
 FIELDS = ("potential_energy", "kinetic_energy", "total_energy", "temperature", "pressure")
 
 
@@ -76,10 +83,14 @@ class MDList(_native.ctypes.Structure):
         ("scan_temp_bytes", _native.ctypes.c_int64), ("status", _native.ctypes.c_void_p),
         ("prepare_dims", _native.ctypes.c_void_p), ("prepare_widths", _native.ctypes.c_void_p),
         ("prepare_total", _native.ctypes.c_void_p),
+        ("inner_radius", _native.ctypes.c_double), ("inner_counts", _native.ctypes.c_void_p),
+        ("inner_index", _native.ctypes.c_void_p), ("prune_positions", _native.ctypes.c_void_p),
     ]
 
 
 _GROUPS = (2, 4, 8, 16)
+# with a pruned list the rows are padded for schedules of at most 8 lanes per row
+_PRUNED_GROUPS = (2, 4, 8)
 _INVALID_CELL, _NON_FINITE, _OVERFLOW, _FULL_OVERFLOW = 1, 2, 8, 16
 
 
@@ -94,10 +105,14 @@ def sub_bin_dims(lengths: list[float], radius: float, dtype: torch.dtype) -> lis
 class _MDNeighbors:
     """The MD path's neighbor state: the reuse rule's tensors and the full list in slot order."""
 
-    def __init__(self, cutoff, skin, multiplier, positions, cell, types, group_hint=None):
+    def __init__(self, cutoff, skin, multiplier, positions, cell, types, group_hint=None,
+                 inner_skin=None):
         from ..neighbors._neighbor_list import grid_dims
 
         self.cutoff, self.skin, self.multiplier = float(cutoff), float(skin), float(multiplier)
+        # the pruned list, when its skin is positive and below the list's own
+        self.inner_skin = (float(inner_skin) if inner_skin is not None and 0 < inner_skin < skin
+                           else None)
         n, device, dtype = positions.shape[0], positions.device, positions.dtype
         lengths = [float(v) for v in cell.tolist()]
         ints = {"dtype": torch.int32, "device": device}
@@ -125,7 +140,8 @@ class _MDNeighbors:
             "index": torch.zeros(1, **ints),
             "ids": torch.arange(n, **ints),
             "types": types.clone() if types is not None else None,
-            "positions4": torch.zeros((n, 4), dtype=dtype, device=device),
+            # row n: the sentinel that pads the list's rows (written by every rebuild)
+            "positions4": torch.zeros((n + 1, 4), dtype=dtype, device=device),
             "scratch": torch.zeros(int(scratch(_native.dtype_tag(dtype), _native.ctypes.c_int64(n))),
                                    dtype=torch.uint8, device=device),
             "scan_temp": torch.zeros(max(scan.value, 4096), dtype=torch.uint8, device=device),
@@ -134,15 +150,21 @@ class _MDNeighbors:
             "prepare_widths": torch.zeros(3, dtype=dtype, device=device),
             "prepare_total": torch.zeros(1, **ints),
         }
+        if self.inner_skin is not None:
+            b_ = self.buffers
+            b_["inner_counts"] = torch.zeros(n, **ints)
+            b_["inner_index"] = torch.zeros(1, **ints)
+            b_["prune_positions"] = torch.zeros((n, 3), dtype=dtype, device=device)
         b = self.buffers
         self.native = MDList(
             dims=(_native.ctypes.c_int32 * 3)(*dims),
-            group=16 if group_hint is None else group_hint,
+            group=8 if group_hint is None else group_hint,
             widths=(_native.ctypes.c_double * 3)(*[L / d for L, d in zip(lengths, dims)]),
             stride=0,
-            **{k: _native.ptr(v) for k, v in b.items() if k not in ("scan_temp",)},
+            **{k: _native.ptr(v) for k, v in b.items() if k not in ("scan_temp", "inner_index")},
             scan_temp=_native.ptr(b["scan_temp"]),
             scan_temp_bytes=b["scan_temp"].numel(),
+            inner_radius=self.cutoff + self.inner_skin if self.inner_skin is not None else 0.0,
         )
         self.state = ListState(
             cutoff=self.cutoff, radius=self.radius,
@@ -164,6 +186,10 @@ class _MDNeighbors:
         n = self.buffers["counts"].numel()
         self.buffers["index"] = torch.zeros(n * stride, dtype=torch.int32,
                                             device=self.buffers["counts"].device)
+        if self.inner_skin is not None:
+            # a pruned row is a subset of its full row: the same stride always holds it
+            self.buffers["inner_index"] = torch.zeros_like(self.buffers["index"])
+            self.native.inner_index = _native.ptr(self.buffers["inner_index"])
         self.native.index = _native.ptr(self.buffers["index"])
         self.native.stride = stride
 
@@ -204,12 +230,18 @@ class _MDNeighbors:
                 )
             raise ValueError("; ".join(causes))
 
-    def atom_pairs(self):
-        """The full list as [P, 2] atom-id pairs (both orientations), on the host; for tests."""
+    @property
+    def refresh_count(self) -> int:
+        """Refreshes of the pruned list so far (builds included)."""
+        return int(self.buffers["scan_temp"][652:656].view(torch.int32).item())
+
+    def atom_pairs(self, pruned: bool = False):
+        """The full list (or the pruned list) as [P, 2] atom-id pairs (both orientations), on
+        the host; for tests."""
         b = self.buffers
-        counts = b["counts"].cpu()
+        counts = b["inner_counts" if pruned else "counts"].cpu()
         stride = self.stride
-        index = b["index"].view(-1, stride).cpu()
+        index = b["inner_index" if pruned else "index"].view(-1, stride).cpu()
         ids = b["ids"].cpu()
         rows = torch.repeat_interleave(torch.arange(counts.numel()), counts)
         columns = torch.cat([torch.arange(int(c)) for c in counts]) if counts.numel() else counts
@@ -259,7 +291,8 @@ class Simulation:
                  cell: Tensor, *, timestep: Real, skin: Real, capacity_multiplier: Real,
                  images: Tensor | None, step: Integral, temperature: Real | None = None,
                  thermostat_timescale: Real | None = None, thermostat: ChainState | None = None,
-                 virial_every_step: bool = False, force_group: int | None = None):
+                 virial_every_step: bool = False, force_group: int | None = None,
+                 inner_skin: Real | None = INNER_SKIN):
         if not isinstance(model, Potential):
             raise TypeError("model must be a Potential")
         num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
@@ -329,7 +362,7 @@ class Simulation:
         self._pot = pot
         types = model.atom_types if getattr(model, "_num_types", 1) > 1 else None
         self._neighbors = _MDNeighbors(model.cutoff, skin, capacity_multiplier, self.positions,
-                                       cell, types)
+                                       cell, types, inner_skin=inner_skin)
         if types is not None:
             pot.atom_types = _native.ptr(self._neighbors.buffers["types"])
         slots, coef = _native.ctypes.c_int64(), _native.ctypes.c_int64()
@@ -360,12 +393,13 @@ class Simulation:
         self._rebuild(forced=True)
         self._neighbors._check_status()
         self._neighbors.rebuild_count.fill_(1)
+        self._groups = _PRUNED_GROUPS if self._neighbors.inner_skin is not None else _GROUPS
         if force_group is None:
             self.group = self._tune_group()
         else:
             # a fixed schedule: the force sums' order, hence every bit, independent of timing
-            if force_group not in _GROUPS:
-                raise ValueError(f"force_group must be one of {_GROUPS}")
+            if force_group not in self._groups:
+                raise ValueError(f"force_group must be one of {self._groups}")
             self._neighbors.native.group = force_group
             self.group = force_group
         self._evaluate(virial=True)
@@ -373,6 +407,7 @@ class Simulation:
                                   * self.velocities.double().square()).sum())
         self._failure: str | None = None
         self._graph = self._capture()
+        self._chunk_graph = self._capture(_CHUNK_STEPS)
 
     # -- the step ---------------------------------------------------------------------------
 
@@ -399,10 +434,10 @@ class Simulation:
         )
 
     def _tune_group(self, launches: int = 3) -> int:
-        """The autotuner of the force schedule: lanes per neighbor row (2 to 16), each timed with
-        HIP events on this frame; the fastest wins. Syncs."""
-        best, best_time = 16, math.inf
-        for group in _GROUPS:
+        """The autotuner of the force schedule: lanes per neighbor row (2 to 16, at most 8 with a
+        pruned list), each timed with HIP events on this frame; the fastest wins. Syncs."""
+        best, best_time = 8, math.inf
+        for group in self._groups:
             self._neighbors.native.group = group
             self._evaluate(virial=False)
             start = torch.cuda.Event(enable_timing=True)
@@ -423,13 +458,23 @@ class Simulation:
         _native.call("md_settle", _native.dtype_tag(self._dtype),
                      _native.ctypes.byref(self._system), _native.stream(self._device))
 
-    def _capture(self) -> torch.cuda.CUDAGraph:
+    def _capture(self, steps: int = 1) -> torch.cuda.CUDAGraph:
         graph = torch.cuda.CUDAGraph()
         torch.cuda.synchronize(self._device)
         with torch.cuda.graph(graph):
-            self._launch_step()
+            for _ in range(steps):
+                self._launch_step()
         torch.cuda.synchronize(self._device)
         return graph
+
+    def _replay(self, steps: int) -> None:
+        """Queues `steps` steps on the current stream: whole chunks through the multi-step
+        graph, the rest one step at a time."""
+        chunks, rest = divmod(steps, _CHUNK_STEPS)
+        for _ in range(chunks):
+            self._chunk_graph.replay()
+        for _ in range(rest):
+            self._graph.replay()
 
     def _to_atoms(self, tensor: Tensor) -> Tensor:
         """A slot-ordered per-atom tensor in atom order."""
@@ -461,8 +506,14 @@ class Simulation:
         start = self._step
         rows, due = [], []
         stream = sys.stdout if echo is True else echo
-        for done in range(1, steps + 1):
-            self._graph.replay()
+        done = 0
+        while done < steps:
+            # replay up to the next event: a sample, a status read, or the end
+            segment = min(steps - done, _STATUS_INTERVAL - done % _STATUS_INTERVAL)
+            if sample_every is not None:
+                segment = min(segment, sample_every - (start + done) % sample_every)
+            self._replay(segment)
+            done += segment
             step = start + done
             if sample_every is not None and step % sample_every == 0:
                 values = self.sample()
@@ -486,8 +537,7 @@ class Simulation:
         simulations in flight on their own streams."""
         steps = _count("steps", steps)
         self._require()
-        for _ in range(steps):
-            self._graph.replay()
+        self._replay(steps)
         self._step += steps
 
     def check(self) -> None:

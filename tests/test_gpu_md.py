@@ -118,8 +118,10 @@ def test_trajectory_matches_oracle(device, dtype, steps, tol, ensemble):
 
 def test_nvt_conserved_energy_band_argon(argon, device):
     """The reference's published NVT run (examples/README.md:58-64): argon_512, fp32, 2,500
-    steps of 4 fs at 120 K, tau 0.1 ps: conserved energy within 0.03 kJ/mol of -1,713.4 kJ/mol,
-    mean T 119.1 K."""
+    steps of 4 fs at 120 K, tau 0.1 ps: "the conserved energy stays within 0.03 kJ/mol of
+    -1,713.4 kJ/mol", mean T 119.1 K. The centre is printed to 0.1 kJ/mol, so the statement
+    allows values in [-1713.48, -1713.32] with a spread of at most 0.06; fp32 runs that round
+    differently (each schedule here) wander differently inside that band."""
     dtype = torch.float32
     x = torch.tensor(argon["positions"], dtype=dtype, device=device)
     v = torch.tensor(argon["velocities"], dtype=dtype, device=device)
@@ -130,8 +132,8 @@ def test_nvt_conserved_energy_band_argon(argon, device):
     samples = sim.run(2500, sample_every=100)
     conserved = samples["conserved_energy"].numpy()
     first = sim  # keep
-    assert abs(conserved[0] - (-1713.4)) < 0.1
-    assert np.abs(conserved - (-1713.4)).max() <= 0.05
+    assert abs(conserved[0] - (-1713.4)) < 0.05
+    assert np.abs(conserved - (-1713.4)).max() <= 0.08
     assert np.ptp(conserved) <= 0.06
     assert abs(samples["temperature"].mean() - 119.1) < 3.0
     assert sim._neighbors._rebuild_count > 10
@@ -178,6 +180,40 @@ def test_md_full_list_pair_set(device, dtype, n, length):
     sim.run(60)
     assert sim._neighbors._rebuild_count > before
     check()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_md_pruned_list_pair_set(device, dtype):
+    """The pruned list the force kernel reads holds exactly the full list's pairs within
+    cutoff + inner skin at its last refresh (both orientations), is refreshed before any atom
+    moves half that skin, and so always holds every pair within the cutoff."""
+    n, length = 4096, 5.8
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    x = on.jittered_lattice(16, length, 0.02, seed=6)
+    masses = torch.full((n,), MASS, dtype=dtype)
+    v = md.maxwell_boltzmann(masses, 300.0, 3)
+    L = torch.full((3,), length, dtype=dtype)
+    sim = md.Simulation(_lj(n, dtype, device), torch.tensor(x, dtype=dtype, device=device),
+                        v.to(device), masses.to(device), L.to(device), timestep=0.004, skin=0.2,
+                        capacity_multiplier=1.25, images=None, step=0, inner_skin=0.05)
+    key = lambda a: a[np.lexsort((a[:, 1], a[:, 0]))]
+    nb = sim._neighbors
+    refreshes = []
+    for _ in range(6):
+        pruned = nb.atom_pairs(pruned=True).numpy()
+        full = {tuple(p) for p in nb.atom_pairs().numpy()}
+        at_refresh = sim._to_atoms(nb.buffers["prune_positions"]).cpu().numpy()
+        brute = on.brute_pairs(at_refresh, L.numpy().astype(npdt), 1.05)
+        expected = np.array([p for p in np.concatenate([brute, brute[:, ::-1]])
+                             if (int(p[0]), int(p[1])) in full])
+        np.testing.assert_array_equal(key(pruned), key(expected))
+        now = on.brute_pairs(sim.state().positions.cpu().numpy(), L.numpy().astype(npdt), 1.0)
+        have = {tuple(p) for p in pruned}
+        assert all((int(i), int(j)) in have for i, j in now)
+        refreshes.append(nb.refresh_count)
+        sim.run(7)
+    assert refreshes[-1] > refreshes[0] + 2  # refreshed between rebuilds too
+    assert refreshes[-1] > nb._rebuild_count
 
 
 def test_graph_replay_equals_eager_launches(device):
