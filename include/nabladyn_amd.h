@@ -29,8 +29,10 @@ enum {
     NDN_STATUS_INVALID_CELL = 1, /* a cell length not finite or at most twice the cutoff */
     NDN_STATUS_NON_FINITE = 2,   /* a coordinate is not finite */
     NDN_STATUS_OVERFLOW = 8,     /* the pairs exceed the capacity; status[1] = pairs needed */
-    NDN_STATUS_FULL_OVERFLOW = 16 /* the full (both-orientation) list exceeds its capacity;
-                                     status[1] is unchanged; only lists with a full list set it */
+    NDN_STATUS_FULL_OVERFLOW = 16, /* the full (both-orientation) list exceeds its capacity;
+                                      status[1] is unchanged; only lists with a full list set it */
+    NDN_STATUS_HALO_OVERFLOW = 32  /* (MD list) a brick's halo exceeds halo_capacity; the
+                                      largest halo is at byte 656 of scan_temp */
 };
 
 /* The device buffers of one neighbor list, in the reference op's argument order
@@ -182,10 +184,17 @@ typedef struct ndn_list_state {
 
 /* The MD step's own neighbor structure: sub-bins at least radius / 2 wide (dims fixed for the
  * run), and a full list (both orientations of every pair within the radius) in fixed-stride rows,
- * one per slot, holding neighbor slots; stride is a multiple of 4 (16-byte row reads). `group`
- * lanes of a wave read one row in the force kernel (2, 4, 8 or 16). status [2] latches errors as the neighbor list's does, with
+ * one per slot. The slots fall into bricks of brick[0] x brick[1] x brick[2] sub-bins (brick[2]
+ * divides 8; brick[1] * brick[2] <= 64; (brick[1] + 4) * (brick[2] + 4) <= 128); a brick's halo
+ * is every sub-bin within 2 of it (periodic), listed once per build in halo_slots, and a row
+ * holds 16-bit local indices into its brick's halo: 0 the sentinel (a point far from every
+ * atom) that pads the row, k >= 1 the slot halo_slots[brick * halo_capacity + k - 1]. Within
+ * each block of 32 entries, sorted entry j sits at (j % 8) * 4 + j / 8 (a lane's 4 entries
+ * together). stride is a multiple of 64. `group` lanes of a wave read one row in the force
+ * kernel (4, 8 or 16). status [2] latches errors as the neighbor list's does, with
  * NDN_STATUS_FULL_OVERFLOW when a row needs more than `stride` entries (status[1] = the width
- * needed). scan_temp holds a 1024-byte header and the scan tiles. */
+ * needed) and NDN_STATUS_HALO_OVERFLOW when a halo needs more than halo_capacity. scan_temp
+ * holds a 1024-byte header and the scan tiles. */
 typedef struct ndn_md_list {
     int32_t dims[3];
     int32_t group;
@@ -197,12 +206,11 @@ typedef struct ndn_md_list {
     int32_t* unordered;         /* [N] */
     int32_t* order;             /* [N] new slot -> previous slot */
     int32_t* counts;            /* [N] neighbors of each slot */
-    int32_t* index;             /* [N * stride] neighbor slots, row-major, each row padded to the
-                                   stride with slot N; NULL while stride is 0 */
+    uint16_t* index;            /* [N * stride] local indices, row-major, each row padded with 0
+                                   to its count rounded up to 32; NULL while stride is 0 */
     int32_t* ids;               /* [N] atom id of each slot */
     int64_t* types;             /* [N] atom type of each slot, or NULL with one type */
-    void* positions4;           /* [N + 1, 4] T: x, y, z, type of each slot; row N the sentinel
-                                   (1e18, 1e18, 1e18) that pads the rows, written by each build */
+    void* positions4;           /* [N + 1, 4] T: x, y, z, type of each slot (row N spare) */
     void* scratch;              /* [ndn_md_scratch_bytes] */
     uint8_t* scan_temp;
     int64_t scan_temp_bytes;
@@ -216,20 +224,32 @@ typedef struct ndn_md_list {
      * rows. inner_index NULL (inner_radius 0) reads the full rows. */
     double inner_radius;
     int32_t* inner_counts;      /* [N] */
-    int32_t* inner_index;       /* [N * stride], rows padded with slot N */
+    uint16_t* inner_index;      /* [N * stride] local indices as in `index`, padded likewise */
     void* prune_positions;      /* [N, 3] T: the frame at the last refresh */
+    /* bricks and their halos */
+    int32_t brick[3];           /* sub-bins per brick along x, y, z */
+    int64_t bricks;             /* ceil(dims[k] / brick[k]) multiplied over k */
+    int64_t halo_capacity;      /* slots per brick in halo_slots; 0 while sizing */
+    int32_t* halo_slots;        /* [bricks * halo_capacity] */
+    int32_t* halo_runs;         /* [bricks * 128] local start of each run of the halo */
+    int32_t* halo_counts;       /* [bricks] */
 } ndn_md_list;
 
 /* Scratch sizes of a system: partial slots and potential-coefficient bytes. */
 int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
                      int64_t* partial_slots, int64_t* coefficient_bytes);
 
+/* The largest halo_capacity the current device's force blocks hold (the halo's positions in
+ * LDS: what one block may take, at most half a CU's). No reference counterpart. */
+int64_t ndn_md_halo_limit(int dtype);
+
 /* Bytes of ndn_md_list.scratch for N atoms. */
 int64_t ndn_md_scratch_bytes(int dtype, int64_t num_atoms);
 
-/* The list build of the MD path: sort into sub-bins, permute the frame, fill the rows (count
- * only while stride is 0), store the reference frame. With `forced` it always runs; otherwise
- * it is gated on list->rebuild as inside a step. */
+/* The list build of the MD path: sort into sub-bins, permute the frame, list the bricks' halos
+ * and fill the rows (count only while stride or halo_capacity is 0), store the reference frame.
+ * With `forced` it always runs (then the pruned rows wait for ndn_md_refresh); otherwise it is
+ * gated on list->rebuild as inside a step. */
 int ndn_md_rebuild(int dtype, const ndn_md_system* system, const ndn_list_state* list,
                    const ndn_md_list* md_list, int forced, ndn_stream_t stream);
 
@@ -255,6 +275,19 @@ int ndn_md_evaluate(int dtype, const ndn_md_system* system, const ndn_potential*
 int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* pot,
                  const ndn_md_list* md_list, int virial, int kick, double timestep,
                  ndn_stream_t stream);
+
+/* E and F of the current frame over the full rows, rewriting the pruned rows from them: the
+ * first evaluation after a forced ndn_md_rebuild when the list has a pruned list (a step
+ * refreshes by itself). The refresh of md/_simulation.py's first frame (md/_simulation.py:177-180);
+ * the reference has no pruned list. */
+int ndn_md_refresh(int dtype, const ndn_md_system* system, const ndn_potential* pot,
+                   const ndn_md_list* md_list, ndn_stream_t stream);
+
+/* The list's rows as slots, for readout and tests: out[slot * stride + k] is the k-th neighbor
+ * of `slot` for k < its count (the pruned rows with `pruned`); the rest is left as it is. No
+ * reference counterpart (the reference's list holds atom indices already). */
+int ndn_md_list_entries(const ndn_md_list* md_list, int64_t num_atoms, int pruned, int32_t* out,
+                        ndn_stream_t stream);
 
 /* Applies the pending closing velocity factor, leaving the velocities as the reference's frame
  * holds them after a step. */

@@ -20,6 +20,7 @@ constexpr int kStatusInvalidCell = NDN_STATUS_INVALID_CELL;
 constexpr int kStatusNonFinite = NDN_STATUS_NON_FINITE;
 constexpr int kStatusOverflow = NDN_STATUS_OVERFLOW;
 constexpr int kStatusFullOverflow = NDN_STATUS_FULL_OVERFLOW;
+constexpr int kStatusHaloOverflow = NDN_STATUS_HALO_OVERFLOW;
 
 inline unsigned int blocks_for(std::int64_t items, int threads = kBlock)
 {

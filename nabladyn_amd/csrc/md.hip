@@ -135,25 +135,29 @@ struct OneTypeLJ
 
     const LJRow<T>* __restrict__ rows;
 
+    // With p = 1 / r^2: u = p^3 (4 eps sigma^12 p^3 - 4 eps sigma^6) - shift and
+    // fs = p^4 (48 eps sigma^12 p^3 - 24 eps sigma^6), the sigma powers folded into the
+    // coefficients once per kernel.
     struct Bound
     {
-        T sigma_squared, four_eps, twentyfour_eps, shift;
+        T e12, e6, f12, f6, shift;
 
         __device__ __forceinline__ void eval(T, T, T r2, T& u, T& fs) const
         {
-            const T inverse = reciprocal(r2);
-            const T s = sigma_squared * inverse;
-            const T p6 = s * s * s;
-            const T p12 = p6 * p6;
-            u = four_eps * (p12 - p6) - shift;
-            fs = twentyfour_eps * ((p12 + p12) - p6) * inverse;
+            const T p = reciprocal(r2);
+            const T p3 = p * p * p;
+            u = p3 * (p3 * e12 - e6) - shift;
+            fs = p3 * (p3 * f12 - f6) * p;
         }
     };
 
     __device__ __forceinline__ Bound bind() const
     {
         const LJRow<T> c = rows[0];
-        return Bound{c.sigma_squared, c.four_eps, c.twentyfour_eps, c.shift};
+        const T s6 = c.sigma_squared * c.sigma_squared * c.sigma_squared;
+        const T s12 = s6 * s6;
+        return Bound{c.four_eps * s12, c.four_eps * s6, (c.twentyfour_eps + c.twentyfour_eps) * s12,
+                     c.twentyfour_eps * s6, c.shift};
     }
 };
 
@@ -250,7 +254,7 @@ struct StepArgs
     index_t* unordered;
     index_t* order;
     index_t* counts;
-    index_t* index;
+    std::uint16_t* index;
     index_t* ids;
     std::int64_t* types;
     T* positions4;
@@ -259,8 +263,15 @@ struct StepArgs
     // the pruned list (inner_index null: none)
     T inner_radius;
     index_t* inner_counts;
-    index_t* inner_index;
+    std::uint16_t* inner_index;
     T* prune_positions;
+    // bricks and their halos
+    index_t brick[3];
+    std::int64_t bricks;
+    std::int64_t halo_capacity;
+    index_t* halo_slots;
+    index_t* halo_runs;
+    index_t* halo_counts;
 };
 
 // The pruned list's slots in the list header (after the 64 max slots): its reuse threshold (the
@@ -277,12 +288,18 @@ __device__ __forceinline__ PruneState* prune_state(ListHeader* header)
     return reinterpret_cast<PruneState*>(reinterpret_cast<std::uint8_t*>(header) + 640);
 }
 
+// The largest halo of the last build (for sizing halo_capacity), after the pruned list's slots.
+__device__ __forceinline__ index_t* halo_needed(ListHeader* header)
+{
+    return reinterpret_cast<index_t*>(reinterpret_cast<std::uint8_t*>(header) + 656);
+}
+
 // Whether the gated refresh of the pruned list runs: after a build (the full rows changed) or
 // when the check asked for it, and no status bit is set.
 template <typename T>
 __device__ __forceinline__ bool refreshing(const StepArgs<T>& a)
 {
-    return a.status[0] == 0 &&
+    return a.inner_index != nullptr && a.status[0] == 0 &&
            (a.rebuild[0] != 0 || prune_state(a.header)->refresh != 0);
 }
 
@@ -366,7 +383,6 @@ __global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T tim
                                                             T half_timestep)
 {
 #pragma clang fp contract(off)
-    using V4 = typename Vec4<T>::type;
     __shared__ T block_max[kBlock / kWave];
     const std::int64_t atom = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool check = a.status[0] == 0;
@@ -424,12 +440,11 @@ __global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T tim
         x[0] = out[0];
         x[1] = out[1];
         x[2] = out[2];
-        V4 p4;
-        p4.x = out[0];
-        p4.y = out[1];
-        p4.z = out[2];
-        p4.w = a.types ? static_cast<T>(a.types[atom]) : T(0);
-        reinterpret_cast<V4*>(a.positions4)[atom] = p4;
+        // the float4 copy's type (w) is written by the build and stays
+        T* p4 = a.positions4 + 4 * atom;
+        p4[0] = out[0];
+        p4[1] = out[1];
+        p4[2] = out[2];
         if (crossed)
         {
             std::int64_t* image = a.images + 3 * atom;
@@ -659,14 +674,204 @@ __device__ __forceinline__ T bin_gap(int offset, T width)
     return steps > 0 ? static_cast<T>(steps) * width : T(0);
 }
 
+// ---- bricks ---------------------------------------------------------------------------------
+//
+// The force kernel works brick by brick. A brick is a box of brick_x x brick_y x brick_z
+// sub-bins (brick_z divides kLayerBlock, so a brick never straddles a layer block), numbered x
+// fastest, then y, then z. Its halo is every sub-bin within 2 of it along each axis (the
+// stencil of each of its sub-bins), periodic, the whole axis where that would reach around it.
+// The halo's atoms are listed once per build, as runs of x-consecutive sub-bins (contiguous
+// slots), and a row of the list names its neighbors by their position in its brick's halo: a
+// 16-bit local index, 0 the sentinel (a far point), 1..H the halo. The force kernel stages the
+// halo's positions in LDS and reads every neighbor there.
+
+constexpr int kMaxBrickRuns = 64;    // brick_y * brick_z: the brick's own x-runs
+constexpr int kMaxHaloRuns = 128;    // (brick_y + 4) * (brick_z + 4) at most
+// Threads of a force block (one block per brick): 1024 for fp32 (16 waves; two blocks fill a CU
+// with the largest halos), 512 for fp64 (whose kernels need up to 256 registers).
+template <typename T>
+constexpr int kBrickThreads = sizeof(T) == 4 ? 1024 : 512;
+
+struct BrickBox
+{
+    index_t lo[3], hi[3];
+};
+
+__device__ __forceinline__ BrickBox brick_box(index_t brick, const index_t dims[3],
+                                              const index_t size[3])
+{
+    const index_t across = (dims[0] + size[0] - 1) / size[0];
+    const index_t down = (dims[1] + size[1] - 1) / size[1];
+    const index_t ix = brick % across;
+    const index_t rest = brick / across;
+    const index_t coords[3] = {ix, rest % down, rest / down};
+    BrickBox box;
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        box.lo[axis] = coords[axis] * size[axis];
+        box.hi[axis] = box.lo[axis] + size[axis] < dims[axis] ? box.lo[axis] + size[axis]
+                                                             : dims[axis];
+    }
+    return box;
+}
+
+__device__ __forceinline__ index_t brick_of(index_t x, index_t y, index_t z, const index_t dims[3],
+                                            const index_t size[3])
+{
+    const index_t across = (dims[0] + size[0] - 1) / size[0];
+    const index_t down = (dims[1] + size[1] - 1) / size[1];
+    return (z / size[2] * down + y / size[1]) * across + x / size[0];
+}
+
+// The halo along one axis: its first sub-bin (before wrapping; -2 or more) and its length.
+__device__ __forceinline__ void halo_axis(index_t lo, index_t hi, index_t n, index_t* start,
+                                          index_t* span)
+{
+    if (hi - lo + 4 >= n)
+    {
+        *start = 0;
+        *span = n;
+    }
+    else
+    {
+        *start = lo - 2;
+        *span = hi - lo + 4;
+    }
+}
+
+__device__ __forceinline__ index_t wrap_index(index_t i, index_t n)
+{
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// The x-range [start, start + span) of a halo run as at most two runs of the row: [a0, a1], then
+// [b0, b1] (empty when b1 < b0) past the x faces.
+__device__ __forceinline__ void x_parts(index_t start, index_t span, index_t n, index_t* a0,
+                                        index_t* a1, index_t* b0, index_t* b1)
+{
+    *b0 = 0;
+    *b1 = -1;
+    if (start < 0)
+    {
+        *a0 = start + n;
+        *a1 = n - 1;
+        *b1 = start + span - 1;
+    }
+    else if (start + span > n)
+    {
+        *a0 = start;
+        *a1 = n - 1;
+        *b1 = start + span - 1 - n;
+    }
+    else
+    {
+        *a0 = start;
+        *a1 = start + span - 1;
+    }
+}
+
+struct Halo
+{
+    index_t xs, sx, ys, sy, zs, sz;
+};
+
+__device__ __forceinline__ Halo brick_halo(const BrickBox& box, const index_t dims[3])
+{
+    Halo h;
+    halo_axis(box.lo[0], box.hi[0], dims[0], &h.xs, &h.sx);
+    halo_axis(box.lo[1], box.hi[1], dims[1], &h.ys, &h.sy);
+    halo_axis(box.lo[2], box.hi[2], dims[2], &h.zs, &h.sz);
+    return h;
+}
+
+// Position of sorted entry k within its row: blocks of 32, in which lane s of 8 holds entries
+// s, s + 8, s + 16 and s + 24 in 4 consecutive places, so that one 8-byte read gives a lane 4
+// entries and the 8 lanes of a row, taking their u-th entries together, read 8 consecutive ones.
+__host__ __device__ __forceinline__ index_t entry_position(index_t k)
+{
+    return (k & ~31) | ((k & 7) << 2) | ((k >> 3) & 3);
+}
+
+// One wave per brick: the runs of its halo, their local starts (halo_runs, 1-based), the slots
+// in local order (halo_slots) and the count. With halo_capacity 0 it only counts. The largest
+// count goes to the header (byte 656) for sizing; a count above the capacity sets
+// NDN_STATUS_HALO_OVERFLOW.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_halo_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const int lane = lane_id();
+    const index_t dims[3] = {a.nx, a.ny, a.nz};
+    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (kBlock / kWave);
+    for (std::int64_t brick =
+             static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+         brick < a.bricks; brick += waves)
+    {
+        const BrickBox box = brick_box(static_cast<index_t>(brick), dims, a.brick);
+        const Halo h = brick_halo(box, dims);
+        index_t a0, a1, b0, b1;
+        x_parts(h.xs, h.sx, a.nx, &a0, &a1, &b0, &b1);
+        const index_t runs = h.sy * h.sz;
+        index_t* starts = a.halo_runs + brick * kMaxHaloRuns;
+        index_t* slots = a.halo_slots + brick * a.halo_capacity;
+        index_t carry = 1;
+        for (index_t base = 0; base < runs; base += kWave)
+        {
+            const index_t q = base + lane;
+            index_t first_a = 0, length_a = 0, first_b = 0, length_b = 0;
+            if (q < runs)
+            {
+                const index_t y = wrap_index(h.ys + q % h.sy, a.ny);
+                const index_t z = wrap_index(h.zs + q / h.sy, a.nz);
+                const std::int64_t row = row_first_bin(y, z, a.nx, a.ny, a.nz);
+                first_a = a.bin_offsets[row + a0];
+                length_a = a.bin_offsets[row + a1 + 1] - first_a;
+                if (b1 >= b0)
+                {
+                    first_b = a.bin_offsets[row + b0];
+                    length_b = a.bin_offsets[row + b1 + 1] - first_b;
+                }
+            }
+            const index_t length = length_a + length_b;
+            const index_t inclusive = wave_inclusive_sum(length);
+            const index_t start = carry + inclusive - length;
+            if (q < runs) starts[q] = start;
+            const index_t here = runs - base < kWave ? runs - base : kWave;
+            for (index_t k = 0; k < here; ++k)
+            {
+                const index_t s = read_lane(start, k);
+                const index_t fa = read_lane(first_a, k);
+                const index_t la = read_lane(length_a, k);
+                const index_t fb = read_lane(first_b, k);
+                const index_t total = la + read_lane(length_b, k);
+                for (index_t t = lane; t < total; t += kWave)
+                {
+                    const index_t place = s - 1 + t;
+                    if (place < a.halo_capacity) slots[place] = t < la ? fa + t : fb + (t - la);
+                }
+            }
+            carry += read_lane(inclusive, kWave - 1);
+        }
+        if (lane == 0)
+        {
+            const index_t count = carry - 1;
+            a.halo_counts[brick] = count;
+            atomicMax(halo_needed(a.header), count);
+            if (a.halo_capacity > 0 && count > a.halo_capacity)
+                atomicOr(a.status, kStatusHaloOverflow);
+        }
+    }
+}
+
 // One wave per sub-bin. Its atoms (the centers) share one stencil: lanes 0..49 hold the slot
 // ranges of the 25 stencil rows (runs of x-consecutive sub-bins; two parts where a run wraps
 // around x), culled by the distance between sub-bins, so every sub-bin a pair within R could
-// touch is kept. The candidates are packed 64 at a time; each chunk is gathered once and tested
-// against every center in turn with the exact minimum-image expression, and each center's hits
-// are ballot-compacted into its row. Rows are then padded to the stride with the sentinel slot
-// N (a far-away point, never within any cutoff), so the force kernel reads whole rows unmasked.
-// With stride 0 it only counts (the first frame, to size the rows).
+// touch is kept, and the offset from a slot of the range to its local index in the brick's halo
+// (each range lies inside one run of the halo). The candidates are packed 64 at a time; each
+// chunk is gathered once and tested against every center in turn with the exact minimum-image
+// expression, and each center's hits are ballot-compacted into its row (at entry_position).
+// Rows are then padded with the sentinel 0 up to their count rounded up to 32, so the force kernel
+// reads whole blocks unmasked. With stride 0 it only counts (the first frame, to size the rows).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius_squared,
                                                          T cull_squared, int forced)
@@ -678,11 +883,12 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
     const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const index_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const index_t dims[3] = {nx, ny, nz};
     const bool full_x = nx < kStencil, full_y = ny < kStencil, full_z = nz < kStencil;
     const int span_x = full_x ? nx : kStencil;
     const int span_y = full_y ? ny : kStencil;
     const int span_z = full_z ? nz : kStencil;
-    const index_t sentinel = a.num_atoms;
+    const bool writing = a.stride > 0;
     // one wave per sub-bin, the waves of the grid striding over the sub-bins
     const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (kBlock / kWave);
     for (std::int64_t bin =
@@ -694,8 +900,12 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
         if (begin == end) continue;
         index_t hx, hy, hz;
         bin_coordinates(bin, nx, ny, nz, &hx, &hy, &hz);
+        const index_t brick = brick_of(hx, hy, hz, dims, a.brick);
+        const Halo h = brick_halo(brick_box(brick, dims, a.brick), dims);
+        index_t a0, a1, b0, b1;
+        x_parts(h.xs, h.sx, nx, &a0, &a1, &b0, &b1);
         // this lane's range: stencil row lane / 2, part lane % 2 of its run of x sub-bins
-        index_t lo = 0, length = 0;
+        index_t lo = 0, length = 0, to_local = 0;
         const int row = lane / 2, part = lane % 2;
         if (row < span_y * span_z)
         {
@@ -745,6 +955,19 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
                     {
                         lo = a.bin_offsets[base + run_lo];
                         length = a.bin_offsets[base + run_hi + 1] - lo;
+                        // the halo run of row (wy, wz) and the part of it this range lies in
+                        if (writing)
+                        {
+                            const index_t q = wrap_index(wz - h.zs, nz) * h.sy +
+                                              wrap_index(wy - h.ys, ny);
+                            const index_t start = a.halo_runs[brick * kMaxHaloRuns + q];
+                            const index_t first_a = a.bin_offsets[base + a0];
+                            if (run_lo >= a0 && run_lo <= a1)
+                                to_local = start - first_a;
+                            else
+                                to_local = start + (a.bin_offsets[base + a1 + 1] - first_a) -
+                                           a.bin_offsets[base + b0];
+                        }
                     }
                 }
             }
@@ -774,6 +997,9 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
                 }
                 const bool valid = t < total;
                 const index_t slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
+                index_t local = slot + __shfl(to_local, r, kWave);
+                // past an overflowing halo's capacity: the sentinel (the status reports it)
+                local = local <= a.halo_capacity ? local : 0;
                 const V4 q = pos[valid ? slot : first];
                 for (int k = 0; k < centers; ++k)
                 {
@@ -800,10 +1026,12 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
                     const bool hit = valid && slot != center && (xx + yy) + zz < radius_squared;
                     const std::uint64_t mask = ballot(hit);
                     const index_t count = read_lane(mine_count, k);
-                    if (hit && a.stride > 0)
+                    if (hit && writing)
                     {
                         const index_t position = count + bits_below(mask);
-                        if (position < a.stride) a.index[center * a.stride + position] = slot;
+                        if (position < a.stride)
+                            a.index[center * a.stride + entry_position(position)] =
+                                static_cast<std::uint16_t>(local);
                     }
                     if (lane == k) mine_count += static_cast<index_t>(__popcll(mask));
                 }
@@ -812,140 +1040,22 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
             if (lane < centers)
             {
                 a.counts[first + lane] = mine_count;
-                if (a.stride > 0 && mine_count > a.stride)
+                if (writing && mine_count > a.stride)
                 {
                     atomicOr(a.status, kStatusFullOverflow);
                     atomicMax(a.status + 1, mine_count);
                 }
             }
-            if (a.stride > 0)
+            if (writing)
                 for (int k = 0; k < centers; ++k)
                 {
                     const index_t count = read_lane(mine_count, k);
-                    index_t* row_out = a.index + static_cast<std::int64_t>(first + k) * a.stride;
-                    for (index_t p = count + lane; p < a.stride; p += kWave) row_out[p] = sentinel;
+                    std::uint16_t* row_out =
+                        a.index + static_cast<std::int64_t>(first + k) * a.stride;
+                    const index_t padded = (count + 31) & ~31;
+                    for (index_t p = count + lane; p < padded && p < a.stride; p += kWave)
+                        row_out[entry_position(p)] = 0;
                 }
-        }
-    }
-}
-
-// Four list entries, read once per step: a non-temporal load, so the list streaming through L2
-// does not evict the positions the gathers reuse.
-typedef index_t Quad __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ Quad stream_quad(const index_t* entries)
-{
-    return __builtin_nontemporal_load(reinterpret_cast<const Quad*>(entries));
-}
-
-// The position of slot j: a 32-bit byte offset from the float4 copy's base (a uniform pointer),
-// which a gather takes in one instruction (the host keeps the copy under 4 GiB).
-template <typename V4>
-__device__ __forceinline__ V4 gather(const V4* __restrict__ pos, index_t j)
-{
-    const std::uint32_t offset =
-        static_cast<std::uint32_t>(j) * static_cast<std::uint32_t>(sizeof(V4));
-    return *reinterpret_cast<const V4*>(reinterpret_cast<const char*>(pos) + offset);
-}
-
-// Refreshes the pruned list from the full rows: four lanes per row, each reading 4 consecutive
-// entries per pass (the group 16), keeping those within the pruned radius (the exact
-// minimum-image test) in row order by ballot compaction. Since every pair kept is in the full
-// row, a pruned row never outgrows the stride. Rows come in aligned groups of 64 (one block); a
-// row is padded with the sentinel slot N up to the group's largest count rounded to 4, and every
-// force schedule's wave (64 / G consecutive rows, aligned) lies inside one such group, so the
-// force kernel never reads past the padding. The frame of the refresh is kept for the check.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) md_prune_kernel(StepArgs<T> a, T inner_radius,
-                                                          T inner_squared, int forced)
-{
-#pragma clang fp contract(off)
-    using V4 = typename Vec4<T>::type;
-    if (!forced && !refreshing(a)) return;
-    constexpr int G = 4;
-    constexpr int kRows = kBlock / G;
-    __shared__ index_t group_max[kBlock / kWave];
-    const int lane = lane_id();
-    const int sub = lane % G;
-    const std::uint64_t mine = 0xFull << (lane - sub);  // the lanes of this row
-    const std::uint64_t below = (1ull << lane) - 1ull;
-    const V4* __restrict__ pos = reinterpret_cast<const V4*>(a.positions4);
-    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
-    const index_t sentinel = a.num_atoms;
-    const std::int64_t groups = (static_cast<std::int64_t>(a.num_atoms) + kRows - 1) / kRows;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
-    for (std::int64_t g = blockIdx.x; g < groups; g += gridDim.x)
-    {
-        const std::int64_t row = g * kRows + threadIdx.x / G;
-        const bool live = row < a.num_atoms;
-        const V4 c = pos[live ? row : 0];
-        // the interior fast path of the force kernel: a wave whose atoms all sit farther than
-        // the pruned radius from every face takes x_i - x_j directly (the minimum image of every
-        // pair that can be kept)
-        const T low = inner_radius + T(1e-3);
-        const bool interior = !live || (c.x > low && c.x < lx - low && c.y > low &&
-                                        c.y < ly - low && c.z > low && c.z < lz - low);
-        const bool fast = ballot(!interior) == 0ull;
-        index_t kept = 0;
-        if (live)
-        {
-            const index_t listed = a.counts[row];
-            const index_t count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
-            const index_t* __restrict__ in_row = a.index + row * a.stride;
-            index_t* __restrict__ out_row = a.inner_index + row * a.stride;
-            // every lane of the row takes the same number of passes (its ballots stay whole)
-            for (index_t pass = 0; pass < count; pass += 4 * G)
-            {
-                const index_t base = pass + 4 * sub;
-                const bool any = base < count;
-                // entries in [count, stride) are the sentinel; base + 3 < stride when base < count
-                const Quad quad = stream_quad(in_row + (any ? base : 0));
-                const index_t js[4] = {quad.x, quad.y, quad.z, quad.w};
-                // all four gathers before any store
-                const V4 q[4] = {gather(pos, js[0]), gather(pos, js[1]), gather(pos, js[2]),
-                                 gather(pos, js[3])};
-                bool hits[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                {
-                    T dx, dy, dz;
-                    if (fast)
-                    {
-                        dx = c.x - q[u].x;
-                        dy = c.y - q[u].y;
-                        dz = c.z - q[u].z;
-                    }
-                    else
-                    {
-                        dx = displacement(c.x, q[u].x, lx);
-                        dy = displacement(c.y, q[u].y, ly);
-                        dz = displacement(c.z, q[u].z, lz);
-                    }
-                    hits[u] = any && exact_squared(dx, dy, dz) < inner_squared;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                {
-                    const std::uint64_t mask = ballot(hits[u]) & mine;
-                    if (hits[u]) out_row[kept + __popcll(mask & below)] = js[u];
-                    kept += static_cast<index_t>(__popcll(mask));
-                }
-            }
-            if (sub < 3) a.prune_positions[3 * row + sub] = a.positions[3 * row + sub];
-            if (sub == 0) a.inner_counts[row] = kept;
-        }
-        // the group's largest count, rounded to 4: the padding every row of the group gets
-        const index_t wave_largest = wave_max(kept);
-        if (lane == 0) group_max[threadIdx.x / kWave] = wave_largest;
-        __syncthreads();
-        index_t largest = group_max[0];
-        for (int w = 1; w < kBlock / kWave; ++w) largest = group_max[w] > largest ? group_max[w] : largest;
-        __syncthreads();
-        const index_t padded = (largest + 3) / 4 * 4;
-        if (live)
-        {
-            index_t* out_row = a.inner_index + row * a.stride;
-            for (index_t p = kept + sub; p < padded; p += G) out_row[p] = sentinel;
         }
     }
 }
@@ -957,233 +1067,499 @@ inline unsigned xcd_grid(std::int64_t blocks)
     return static_cast<unsigned>((blocks + 7) / 8 * 8);
 }
 
+// The value of lane (lane ^ 1), (lane ^ 2) within its quad, or of lane 7 - i (i the lane within
+// its 8) or 15 - i (within its 16): data-parallel moves, no LDS round trip.
+template <int kControl>
+__device__ __forceinline__ float dpp_move(float value)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(value), kControl, 0xF, 0xF, false));
+}
+
+template <int kControl>
+__device__ __forceinline__ double dpp_move(double value)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(value), kControl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(value), kControl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+constexpr int kDppSwapOne = 0xB1;      // quad_perm [1, 0, 3, 2]
+constexpr int kDppSwapTwo = 0x4E;      // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // lane 7 - i within each 8
+constexpr int kDppMirror = 0x140;      // lane 15 - i within each 16
+
+// The sum over each aligned group of G lanes (G = 2, 4, 8 or 16), the same bits on every lane of
+// the group (each step adds the same two partial sums, a + b on one lane and b + a on the other).
 template <int G, typename V>
 __device__ __forceinline__ V group_sum(V value)
 {
-#pragma unroll
-    for (int offset = G / 2; offset > 0; offset >>= 1) value += __shfl_xor(value, offset, kWave);
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16, "groups of 2 to 16 lanes");
+    value += dpp_move<kDppSwapOne>(value);
+    if (G >= 4) value += dpp_move<kDppSwapTwo>(value);
+    if (G >= 8) value += dpp_move<kDppHalfMirror>(value);
+    if (G >= 16) value += dpp_move<kDppMirror>(value);
     return value;
 }
 
-
-
-// One group's walk over its row: each lane takes 4 consecutive entries (one 16-byte load) per
-// pass, the group 4G entries, and the lane's four position gathers are issued together. A row is
-// padded to the stride with the sentinel slot N, a point far from every atom, so entries past
-// the row's count fail the cutoff test by themselves and need no mask. The row start is 16-byte
-// aligned and `width` <= stride, a multiple of 8, so no read leaves the row.
-//
-// In refresh mode (the step that refreshes the pruned list) the walk reads a full row and also
-// keeps its pairs within the pruned radius: each lane's four are ballot-compacted, in row order,
-// into the pruned row `kept_row`, and `kept` counts them (the same on every lane of the group:
-// every lane takes the same passes).
-template <int G, bool kFast, bool kVirial, bool kRefresh = false, typename T, typename V4,
-          typename Ev>
-__device__ __forceinline__ void walk_row(const Ev& ev, const V4& c, const V4* __restrict__ pos,
-                                         const index_t* __restrict__ entries, index_t width,
-                                         T lx, T ly, T lz, T cutoff_squared, T& fx, T& fy, T& fz,
-                                         T& energy, T* w, T inner_squared = T(0),
-                                         index_t* __restrict__ kept_row = nullptr,
-                                         index_t* kept = nullptr)
+// The largest value over each aligned group of G lanes.
+template <int G>
+__device__ __forceinline__ index_t group_max(index_t value)
 {
-    const int lane = lane_id();
-    const std::uint64_t group = ((1ull << G) - 1ull) << (lane - lane % G);
-    const std::uint64_t below = (1ull << lane) - 1ull;
-    // A plain walk lets each lane stop at its own last pass. A refresh carries the compaction
-    // count across the lanes of a row, so there every lane takes the same passes: a lane whose
-    // entries start at or past `width` reads the row's first entries (valid slots) and keeps
-    // and counts nothing.
-    const index_t start = kRefresh ? 0 : 4 * (lane % G);
-    const index_t offset = kRefresh ? 4 * (lane % G) : 0;
-    for (index_t pass = start; pass < width; pass += 4 * G)
-    {
-        const index_t base = pass + offset;
-        const bool any = !kRefresh || base < width;
-        const Quad quad = stream_quad(entries + (any ? base : 0));
-        const V4 q[4] = {gather(pos, quad.x), gather(pos, quad.y), gather(pos, quad.z),
-                         gather(pos, quad.w)};
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16, "groups of 2 to 16 lanes");
+    auto take = [](index_t v, index_t other) { return other > v ? other : v; };
+    value = take(value, __builtin_amdgcn_update_dpp(0, value, kDppSwapOne, 0xF, 0xF, false));
+    if (G >= 4) value = take(value, __builtin_amdgcn_update_dpp(0, value, kDppSwapTwo, 0xF, 0xF, false));
+    if (G >= 8) value = take(value, __builtin_amdgcn_update_dpp(0, value, kDppHalfMirror, 0xF, 0xF, false));
+    if (G >= 16) value = take(value, __builtin_amdgcn_update_dpp(0, value, kDppMirror, 0xF, 0xF, false));
+    return value;
+}
+
+// Four entries of a row, read once per step: one 8-byte non-temporal load, so the list streaming
+// through L2 does not evict the positions the halo loads reuse.
+typedef unsigned Pair2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ Pair2 stream_entries(const std::uint16_t* entries)
+{
+    return __builtin_nontemporal_load(reinterpret_cast<const Pair2*>(entries));
+}
+
+// The pairs of four entries of a row (one lane's 8-byte read): positions from the halo in LDS,
+// the minimum image (or the direct difference on a fast wave), the cutoff mask and the sums.
+template <bool kFast, bool kVirial, bool kEnergy, typename T, typename V4, typename Ev>
+__device__ __forceinline__ void walk_quad(const Ev& ev, const V4& c, const V4* halo, Pair2 raw,
+                                          bool any, T lx, T ly, T lz, T cutoff_squared, T& fx,
+                                          T& fy, T& fz, T& energy, T* w)
+{
+    const unsigned local[4] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16};
+    const V4 q[4] = {halo[local[0]], halo[local[1]], halo[local[2]], halo[local[3]]};
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 4; ++u)
+    {
+        T dx, dy, dz;
+        if (kFast)
         {
-            T dx, dy, dz;
-            if (kFast)
-            {
-                dx = c.x - q[u].x;
-                dy = c.y - q[u].y;
-                dz = c.z - q[u].z;
-            }
-            else
-            {
-                dx = displacement(c.x, q[u].x, lx);
-                dy = displacement(c.y, q[u].y, ly);
-                dz = displacement(c.z, q[u].z, lz);
-            }
-            const T r2 = exact_squared(dx, dy, dz);
-            const bool inside = any && r2 < cutoff_squared;
-            if (kRefresh)
-            {
-                const bool keep = any && r2 < inner_squared;
-                const std::uint64_t mask = ballot(keep) & group;
-                if (keep) kept_row[*kept + __popcll(mask & below)] = (u == 0 ? quad.x : u == 1 ? quad.y : u == 2 ? quad.z : quad.w);
-                *kept += static_cast<index_t>(__popcll(mask));
-            }
-            T pu, fs;
-            ev.eval(c.w, q[u].w, r2, pu, fs);
-            // a select, not a product: a masked pair's fs may be inf or NaN
-            pu = inside ? pu : T(0);
-            fs = inside ? fs : T(0);
-            energy += pu;
-            fx += fs * dx;
-            fy += fs * dy;
-            fz += fs * dz;
-            if (kVirial)
-            {
-                w[0] += dx * (fs * dx);
-                w[1] += dy * (fs * dy);
-                w[2] += dz * (fs * dz);
-                w[3] += dx * (fs * dy);
-                w[4] += dx * (fs * dz);
-                w[5] += dy * (fs * dz);
-            }
+            dx = c.x - q[u].x;
+            dy = c.y - q[u].y;
+            dz = c.z - q[u].z;
+        }
+        else
+        {
+            dx = displacement(c.x, q[u].x, lx);
+            dy = displacement(c.y, q[u].y, ly);
+            dz = displacement(c.z, q[u].z, lz);
+        }
+        const T r2 = exact_squared(dx, dy, dz);
+        const bool inside = any && r2 < cutoff_squared;
+        T pu, fs;
+        ev.eval(c.w, q[u].w, r2, pu, fs);
+        // a select, not a product: a masked pair's fs may be inf or NaN
+        fs = inside ? fs : T(0);
+        if (kEnergy) energy += inside ? pu : T(0);
+        fx += fs * dx;
+        fy += fs * dy;
+        fz += fs * dz;
+        if (kVirial)
+        {
+            w[0] += dx * (fs * dx);
+            w[1] += dy * (fs * dy);
+            w[2] += dz * (fs * dz);
+            w[3] += dx * (fs * dy);
+            w[4] += dx * (fs * dz);
+            w[5] += dy * (fs * dz);
         }
     }
 }
 
-// Waves per SIMD the force kernel is compiled for: each potential's `waves`, the most its fp32
-// kernels without the virial reach without spilling (one-type LJ 8, typed LJ 5, table 4). The virial's six more sums, and fp64, need more
-// registers: those are left to the compiler.
-template <typename T, class Pot, bool kVirial, int kMode>
-constexpr int kForceWaves = kVirial || kMode == 2 ? 1 : Pot::template waves<T>;
-
-// The full list's rows in tiles of kBlock / G rows, G lanes per row. The grid is persistent: the
-// blocks of one XCD (blockIdx % 8; blocks are dealt to the 8 XCDs round-robin) walk a contiguous
-// eighth of the tiles, so neighboring rows, which gather the same positions, share that XCD's
-// L2. Each block sums its E, W and K in a fixed order into its partial slot, so every run gives
-// the same bits.
-//
-// Images are chosen once per wave: a wave whose atoms all sit farther than the cutoff (plus a
-// margin) from every face takes x_i - x_j directly, which is the minimum image for every pair
-// within the cutoff; any other image it may pick is longer than L/2 > rc, so the mask drops it
-// exactly as the reference does. Other waves use the reference's displacement (space.py:11-19)
-// per pair.
-// kMode: 0 reads the pruned rows (the full rows when there are none); in a step with a pruned
-// list two launches share the work, each gated on the device's refresh flag: mode 1 reads the
-// pruned rows and runs when there is no refresh, mode 2 runs when there is one, reading the full
-// rows and writing the pruned rows, their counts (each row padded with the sentinel up to the
-// tile's largest count rounded to 4; tiles are at least 32 aligned rows, and so hold every wave
-// of every schedule of at most 8 lanes per row) and the frame of the refresh.
-//
-// The partial slots past this launch's grid, up to `slots`, are zeroed: the two launches of a
-// step may have different grids and the finish reads the larger.
-template <typename T, class Pot, int G, bool kVirial, bool kKick, int kMode = 0>
-__global__ void __launch_bounds__(kBlock, (kForceWaves<T, Pot, kVirial, kMode>)) force_kernel(StepArgs<T> a, Pot pot, T cutoff,
-                                                       T cutoff_squared, T half_timestep,
-                                                       double* __restrict__ partials,
-                                                       index_t tiles, index_t slots)
+// One group's walk over its row: lane s of the row's G lanes takes the 4 entries at 4G m + 4s
+// (one 8-byte read) for m = 0, 1, ..., two at a time with both reads issued first (the second's
+// latency hides behind the first's arithmetic), up to `limit`, the row's count rounded up to a
+// block of 32. A row is padded with the sentinel 0, a point far from every atom, so entries past
+// the count fail the cutoff test by themselves and need no mask.
+template <int G, bool kFast, bool kVirial, bool kEnergy, typename T, typename V4, typename Ev>
+__device__ __forceinline__ void walk_row(const Ev& ev, const V4& c, const V4* halo,
+                                         const std::uint16_t* __restrict__ entries, index_t limit,
+                                         T lx, T ly, T lz, T cutoff_squared, T& fx, T& fy, T& fz,
+                                         T& energy, T* w)
 {
-    using V4 = typename Vec4<T>::type;
-    constexpr bool kRefresh = kMode == 2;
-    if (kMode == 1 && refreshing(a)) return;
-    if (kMode == 2 && !refreshing(a)) return;
-    __shared__ double shared[16];
-    __shared__ index_t tile_largest[kBlock / kWave];
-    constexpr int kRows = kBlock / G;
-    const auto ev = pot.bind();
-    const int sub = lane_id() % G;
-    const V4* __restrict__ pos = reinterpret_cast<const V4*>(a.positions4);
-    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
-    // a refresh keeps pairs out to the pruned radius: the direct difference must be their
-    // minimum image too
-    const T low = (kRefresh ? a.inner_radius : cutoff) + T(1e-3);
-    const T inner_squared = kRefresh ? a.inner_radius * a.inner_radius : T(0);
-    if (kRefresh && blockIdx.x == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
-    const unsigned xcd = blockIdx.x % 8;
-    const index_t first = static_cast<index_t>(static_cast<std::int64_t>(tiles) * xcd / 8);
-    const index_t last = static_cast<index_t>(static_cast<std::int64_t>(tiles) * (xcd + 1) / 8);
-    const index_t step = static_cast<index_t>(gridDim.x / 8);
-    double energy_sum = 0.0, kinetic = 0.0;
-    double w_sum[6] = {0, 0, 0, 0, 0, 0};
-    for (index_t tile = first + static_cast<index_t>(blockIdx.x / 8); tile < last; tile += step)
+    for (index_t first = 4 * (lane_id() % G); first < limit; first += 8 * G)
     {
-        const index_t row = tile * kRows + static_cast<index_t>(threadIdx.x) / G;
-        const bool live = row < a.num_atoms;
-        // a row past the last atom walks row 0's entries from a far point of its own (apart from
-        // the sentinel too), so none of its pairs is within the cutoff
-        V4 c = pos[live ? row : 0];
-        if (!live) c.x = c.y = c.z = -kFarAway<T>;
-        // the pruned rows when there are any (not on a refresh); a full row that overflowed
-        // holds only `stride` entries (the status bit reports it)
-        const bool pruned = !kRefresh && a.inner_index != nullptr;
-        const index_t* counts = pruned ? a.inner_counts : a.counts;
-        const index_t listed = live ? counts[row] : 0;
-        const index_t count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
-        const bool interior = live && c.x > low && c.x < lx - low && c.y > low &&
-                              c.y < ly - low && c.z > low && c.z < lz - low;
-        const bool fast = ballot(!interior) == 0ull;
-        const index_t width = wave_max(count);
-        const std::int64_t row_start = static_cast<std::int64_t>(live ? row : 0) * a.stride;
-        const index_t* entries = (pruned ? a.inner_index : a.index) + row_start;
-        T fx = 0, fy = 0, fz = 0, energy = 0;
-        T w[6] = {0, 0, 0, 0, 0, 0};
-        index_t kept = 0;
-        index_t* kept_row = kRefresh ? a.inner_index + row_start : nullptr;
-        if (fast)
-            walk_row<G, true, kVirial, kRefresh>(ev, c, pos, entries, width, lx, ly, lz,
-                                                 cutoff_squared, fx, fy, fz, energy, w,
-                                                 inner_squared, kept_row, &kept);
-        else
-            walk_row<G, false, kVirial, kRefresh>(ev, c, pos, entries, width, lx, ly, lz,
-                                                  cutoff_squared, fx, fy, fz, energy, w,
-                                                  inner_squared, kept_row, &kept);
-        if (kRefresh)
+        const index_t second = first + 4 * G;
+        const bool any_second = second < limit;
+        const Pair2 raw_first = stream_entries(entries + first);
+        const Pair2 raw_second = stream_entries(entries + (any_second ? second : first));
+        // both reads stay here: the scheduler would otherwise sink the second to its use
+        __builtin_amdgcn_sched_barrier(0);
+        walk_quad<kFast, kVirial, kEnergy>(ev, c, halo, raw_first, true, lx, ly, lz,
+                                           cutoff_squared, fx, fy, fz, energy, w);
+        // past the row's end the second quad is all masked (no branch: the wave walks as far as
+        // its longest row anyway)
+        walk_quad<kFast, kVirial, kEnergy>(ev, c, halo, raw_second, any_second, lx, ly, lz,
+                                           cutoff_squared, fx, fy, fz, energy, w);
+    }
+}
+
+// The block of brick `b`: blocks are dealt to the 8 XCDs round-robin (blockIdx % 8), and the
+// blocks of one XCD take a contiguous eighth of the bricks, so bricks side by side, whose halos
+// overlap, share that XCD's L2. Blocks past the last brick idle.
+__device__ __forceinline__ index_t xcd_brick(index_t bricks)
+{
+    const index_t per_xcd = (bricks + 7) / 8;
+    return static_cast<index_t>(blockIdx.x % 8) * per_xcd + static_cast<index_t>(blockIdx.x / 8);
+}
+
+// The LDS of a brick's block, one dynamic allocation: the halo (positions and types; entry 0 the
+// far point) from offset 0, so a neighbor's address is its local index times 16 (32 in fp64),
+// then the brick's runs and the block's sums.
+struct BrickShared
+{
+    index_t run_first[kMaxBrickRuns];
+    index_t run_rows[kMaxBrickRuns + 1];
+    double sums[16];
+};
+
+// Bytes of a block's LDS past the halo.
+constexpr std::size_t kBrickSharedBytes = (sizeof(BrickShared) + 15) / 16 * 16;
+
+template <typename T>
+__host__ __device__ __forceinline__ std::size_t halo_bytes(std::int64_t halo_capacity)
+{
+    return static_cast<std::size_t>(halo_capacity + 1) * sizeof(typename Vec4<T>::type);
+}
+
+template <typename T>
+__device__ __forceinline__ BrickShared* brick_shared(unsigned char* lds, std::int64_t halo_capacity)
+{
+    return reinterpret_cast<BrickShared*>(lds + halo_bytes<T>(halo_capacity));
+}
+
+__device__ __forceinline__ index_t brick_row_slot(index_t r, index_t runs,
+                                                  const index_t* run_first,
+                                                  const index_t* run_rows)
+{
+    int run = 0;
+    for (int step = kMaxBrickRuns / 2; step > 0; step >>= 1)
+        if (run + step < runs && run_rows[run + step] <= r) run += step;
+    return run_first[run] + (r - run_rows[run]);
+}
+
+// The periodic frame of a brick. Along each axis every position p is replaced by its image nearest
+// the brick, written so that a plain difference of two images is bit for bit the reference's
+// minimum-image displacement (space.py:11-19) of every pair within `reach` of the brick's atoms:
+// a brick in the lower half keeps p, or takes p - L for an atom more than L / 2 above its center
+// (then p > L / 2, so p - L is exact, and c - (p - L) is the reference's c - (o - L)); a brick
+// whose halo passes the upper face moves its whole frame down by L: p - L for every atom but
+// those more than L / 2 below its center, which keep p (p - L is exact for all atoms within
+// reach, which lie in the upper half, and (c - L) - p is the reference's (c - L) - o). The frame
+// exists only when every atom within reach of the brick lies within L / 2 of its center (and, for
+// the moved frame, in the upper half); otherwise `exact` is false and the walk takes the
+// reference's displacement per pair on unmoved positions.
+template <typename T>
+struct BrickFrame
+{
+    T center[3], length[3];
+    bool moved[3];
+    bool exact;
+
+    __device__ __forceinline__ T image(T p, int axis) const
+    {
+        const T offset = p - center[axis];
+        const T half = T(0.5) * length[axis];
+        if (moved[axis]) return offset < -half ? p : p - length[axis];
+        return offset > half ? p - length[axis] : p;
+    }
+
+    template <typename V4>
+    __device__ __forceinline__ V4 place(V4 p) const
+    {
+        if (!exact) return p;
+        p.x = image(p.x, 0);
+        p.y = image(p.y, 1);
+        p.z = image(p.z, 2);
+        return p;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const BrickBox& box,
+                                                     T reach)
+{
+    BrickFrame<T> f;
+    f.exact = true;
+    const T widths[3] = {a.wx, a.wy, a.wz};
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        const T w = widths[axis];
+        const T lo = static_cast<T>(box.lo[axis]) * w;
+        const T hi = static_cast<T>(box.hi[axis]) * w;
+        const T length = a.cell[axis];
+        f.center[axis] = T(0.5) * (lo + hi);
+        f.length[axis] = length;
+        // the brick's atoms may have left its sub-bins by up to half the list's skin since the
+        // build; a sub-bin of margin (at least half the list's radius) covers that and the
+        // rounding of the sub-bin bounds
+        const T extent = T(0.5) * (hi - lo) + reach + w;
+        f.moved[axis] = f.center[axis] + extent > length;
+        const bool inside = extent < T(0.5) * length;
+        const bool upper = !f.moved[axis] || f.center[axis] - extent > T(0.5) * length;
+        f.exact = f.exact && inside && upper;
+    }
+    return f;
+}
+
+// A brick staged by its block: the halo in LDS, placed in the brick's frame, and the brick's
+// x-runs (y outer, z inner: slot order) with their first rows; row r of the walk is slot_of(r).
+// (Walking the rows sorted by length instead measured slower: rows of one wave then come from
+// all over the brick.)
+template <typename T>
+struct StagedBrick
+{
+    index_t runs;
+    index_t rows;
+    const BrickShared* shared;
+    BrickFrame<T> frame;
+
+    __device__ __forceinline__ index_t slot_of(index_t r) const
+    {
+        return brick_row_slot(r, runs, shared->run_first, shared->run_rows);
+    }
+};
+
+template <int kThreads, typename T, typename V4>
+__device__ __forceinline__ StagedBrick<T> stage_brick(const StepArgs<T>& a, index_t brick,
+                                                      V4* halo, BrickShared* sh, T reach)
+{
+    const index_t dims[3] = {a.nx, a.ny, a.nz};
+    const BrickBox box = brick_box(brick, dims, a.brick);
+    const BrickFrame<T> frame = brick_frame(a, box, reach);
+    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
+    // an overflowing halo (the status reports it) loads what its slots hold
+    const index_t listed = a.halo_counts[brick];
+    const index_t halo_size = listed < a.halo_capacity ? listed : a.halo_capacity;
+    const index_t* slots = a.halo_slots + static_cast<std::int64_t>(brick) * a.halo_capacity;
+    for (index_t k = threadIdx.x; k < halo_size; k += kThreads)
+        halo[k + 1] = frame.place(pos[slots[k]]);
+    if (threadIdx.x == 0)
+    {
+        V4 far;
+        far.x = far.y = far.z = kFarAway<T>;
+        far.w = T(0);
+        halo[0] = far;
+    }
+    const index_t layers = box.hi[2] - box.lo[2];
+    const index_t runs = (box.hi[1] - box.lo[1]) * layers;
+    if (threadIdx.x < kWave)
+    {
+        const int lane = threadIdx.x;
+        index_t length = 0;
+        if (lane < runs)
         {
-            // the tile's largest pruned count, rounded to 4: every row's padding
-            const index_t wave_largest = wave_max(live ? kept : 0);
-            if (lane_id() == 0) tile_largest[threadIdx.x / kWave] = wave_largest;
-            __syncthreads();
-            index_t largest = tile_largest[0];
-            for (int k = 1; k < kBlock / kWave; ++k)
-                largest = tile_largest[k] > largest ? tile_largest[k] : largest;
-            __syncthreads();
-            const index_t padded = (largest + 3) / 4 * 4;
-            if (live)
+            const std::int64_t row = row_first_bin(box.lo[1] + lane / layers,
+                                                   box.lo[2] + lane % layers, a.nx, a.ny, a.nz);
+            const index_t first = a.bin_offsets[row + box.lo[0]];
+            length = a.bin_offsets[row + box.hi[0]] - first;
+            sh->run_first[lane] = first;
+        }
+        const index_t inclusive = wave_inclusive_sum(length);
+        if (lane < runs) sh->run_rows[lane + 1] = inclusive;
+        if (lane == 0) sh->run_rows[0] = 0;
+    }
+    __syncthreads();
+    return StagedBrick<T>{runs, sh->run_rows[runs], sh, frame};
+}
+
+// Lanes per row of the prune kernel: 16, each reading 4 entries per pass.
+constexpr int kPruneGroup = 16;
+
+// Refreshes the pruned list from the full rows, one block per brick (the halo in LDS): 16 lanes
+// per row test the row's pairs against the pruned radius with the exact minimum-image expression
+// and write those kept from the row's start in pass order (ballot ranks, so each store is one
+// contiguous run per row), padded with the sentinel 0 up to a multiple of 32; the count is the
+// number kept. Since every pair kept is in the full row, a pruned row never outgrows the stride.
+// The frame of the refresh is kept for the check. Gated on the refresh flag unless `forced`.
+template <typename T>
+__global__ void __launch_bounds__(kBrickThreads<T>) md_prune_kernel(StepArgs<T> a, int forced)
+{
+#pragma clang fp contract(off)
+    using V4 = typename Vec4<T>::type;
+    if (!forced && !refreshing(a)) return;
+    extern __shared__ __align__(16) unsigned char dynamic_shared[];
+    V4* halo = reinterpret_cast<V4*>(dynamic_shared);
+    BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
+    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks));
+    if (brick >= a.bricks) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
+    // images exact out to the pruned radius
+    const StagedBrick<T> staged =
+        stage_brick<kBrickThreads<T>>(a, brick, halo, sh, a.inner_radius);
+    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
+    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
+    const T inner_squared = a.inner_radius * a.inner_radius;
+    const bool fast = staged.frame.exact;
+    const int lane = lane_id();
+    const int sub = lane % kPruneGroup;
+    const std::uint64_t group = ((1ull << kPruneGroup) - 1ull) << (lane - sub);
+    const std::uint64_t below = (1ull << lane) - 1ull;
+    constexpr index_t kRowsPerPass = kBrickThreads<T> / kPruneGroup;
+    for (index_t first_row = 0; first_row < staged.rows; first_row += kRowsPerPass)
+    {
+        const index_t r = first_row + static_cast<index_t>(threadIdx.x) / kPruneGroup;
+        const bool live = r < staged.rows;
+        const index_t slot = live ? staged.slot_of(r) : 0;
+        V4 c = staged.frame.place(pos[slot]);
+        if (!live) c.x = c.y = c.z = -kFarAway<T>;
+        const index_t listed = live ? a.counts[slot] : 0;
+        const index_t count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
+        const index_t limit = (count + 31) & ~31;
+        const std::int64_t row_start = static_cast<std::int64_t>(slot) * a.stride;
+        const std::uint16_t* entries = a.index + row_start;
+        std::uint16_t* out_row = a.inner_index + row_start;
+        index_t kept = 0;
+        // every lane of a row takes the same passes (the rank carries across them)
+        for (index_t pass = 0; pass < limit; pass += 4 * kPruneGroup)
+        {
+            const index_t base = pass + 4 * sub;
+            const bool any = base < limit;
+            const Pair2 raw = stream_entries(entries + (any ? base : 0));
+            const unsigned local[4] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu,
+                                       raw.y >> 16};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
             {
-                for (index_t p = kept + sub; p < padded; p += G) kept_row[p] = a.num_atoms;
-                if (sub == 0) a.inner_counts[row] = kept;
-                if (sub < 3) a.prune_positions[3LL * row + sub] = a.positions[3LL * row + sub];
+                const V4 q = halo[local[u]];
+                T dx, dy, dz;
+                if (fast)
+                {
+                    dx = c.x - q.x;
+                    dy = c.y - q.y;
+                    dz = c.z - q.z;
+                }
+                else
+                {
+                    dx = minimum_image(c.x, q.x, lx);
+                    dy = minimum_image(c.y, q.y, ly);
+                    dz = minimum_image(c.z, q.z, lz);
+                }
+                const T xx = dx * dx;
+                const T yy = dy * dy;
+                const T zz = dz * dz;
+                const bool keep = any && (xx + yy) + zz < inner_squared;
+                const std::uint64_t mask = ballot(keep) & group;
+                if (keep)
+                    out_row[kept + __popcll(mask & below)] = static_cast<std::uint16_t>(local[u]);
+                kept += static_cast<index_t>(__popcll(mask));
             }
         }
+        if (live)
+        {
+            for (index_t p = kept + sub; p < ((kept + 31) & ~31); p += kPruneGroup) out_row[p] = 0;
+            if (sub == 0) a.inner_counts[slot] = kept;
+            if (sub < 3) a.prune_positions[3LL * slot + sub] = a.positions[3LL * slot + sub];
+        }
+    }
+}
+
+// Waves per SIMD the force kernel is compiled for: each potential's `waves` (what its fp32
+// kernels without the virial reach without spilling). The virial's six more sums and fp64 need
+// more registers: those are left to the compiler (a block of kBrickThreads keeps at least 2
+// waves per SIMD).
+template <typename T, class Pot, bool kVirial>
+constexpr int kForceWaves = kVirial ? 1 : Pot::template waves<T>;
+
+// One block per brick: the brick's halo goes to LDS, then G lanes per row walk its rows,
+// kBrickThreads / G rows at a time, in slot order along the brick's x-runs, each row up to its
+// own count rounded up to 32; the pruned rows when there are any. Each block sums its E, W and
+// K in a fixed order into its partial slot, so every run gives the same bits.
+//
+// Images come from the brick's frame (BrickFrame): x_i - x_j of the placed positions is the
+// reference's minimum-image displacement for every pair within the cutoff, and any other pair is
+// farther than the cutoff either way, so the mask drops it exactly as the reference does. A brick
+// without a frame (a box short against the cutoff) takes the reference's displacement
+// (space.py:11-19) per pair. The kick steps skip E unless they also sum W (nothing in a step
+// reads E).
+template <typename T, class Pot, int G, bool kVirial, bool kKick>
+__global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial>))
+    force_kernel(StepArgs<T> a, Pot pot, T cutoff, T cutoff_squared, T half_timestep,
+                 double* __restrict__ partials)
+{
+    using V4 = typename Vec4<T>::type;
+    constexpr bool kEnergy = !kKick || kVirial;
+    extern __shared__ __align__(16) unsigned char dynamic_shared[];
+    V4* halo = reinterpret_cast<V4*>(dynamic_shared);
+    BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
+    double* out = partials + 8LL * blockIdx.x;
+    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks));
+    if (brick >= a.bricks)
+    {
+        if (threadIdx.x < 8) out[threadIdx.x] = 0.0;
+        return;
+    }
+    // the pruned rows when there are any; a full row that overflowed holds only `stride`
+    // entries (the status bit reports it)
+    const bool pruned = a.inner_index != nullptr;
+    const index_t* counts = pruned ? a.inner_counts : a.counts;
+    const std::uint16_t* index = pruned ? a.inner_index : a.index;
+    // images exact out to the cutoff
+    const StagedBrick<T> staged = stage_brick<kBrickThreads<T>>(a, brick, halo, sh, cutoff);
+    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
+    const int sub = lane_id() % G;
+    const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
+    const bool fast = staged.frame.exact;
+    const auto ev = pot.bind();
+    double energy_sum = 0.0, kinetic = 0.0;
+    double w_sum[6] = {0, 0, 0, 0, 0, 0};
+    constexpr index_t kRowsPerPass = kBrickThreads<T> / G;
+    for (index_t first_row = 0; first_row < staged.rows; first_row += kRowsPerPass)
+    {
+        const index_t r = first_row + static_cast<index_t>(threadIdx.x) / G;
+        const bool live = r < staged.rows;
+        const index_t slot = live ? staged.slot_of(r) : 0;
+        // a row past the brick's last walks nothing
+        V4 c = staged.frame.place(pos[slot]);
+        if (!live) c.x = c.y = c.z = -kFarAway<T>;
+        const index_t listed_row = live ? counts[slot] : 0;
+        const index_t count = listed_row < a.stride ? listed_row : static_cast<index_t>(a.stride);
+        const index_t limit = (count + 31) & ~31;
+        const std::uint16_t* entries = index + static_cast<std::int64_t>(slot) * a.stride;
+        T fx = 0, fy = 0, fz = 0, energy = 0;
+        T w[6] = {0, 0, 0, 0, 0, 0};
+        if (fast)
+            walk_row<G, true, kVirial, kEnergy>(ev, c, halo, entries, limit, lx, ly, lz,
+                                                cutoff_squared, fx, fy, fz, energy, w);
+        else
+            walk_row<G, false, kVirial, kEnergy>(ev, c, halo, entries, limit, lx, ly, lz,
+                                                 cutoff_squared, fx, fy, fz, energy, w);
         fx = group_sum<G>(fx);
         fy = group_sum<G>(fy);
         fz = group_sum<G>(fz);
-        energy_sum += static_cast<double>(energy);
+        if (kEnergy) energy_sum += static_cast<double>(energy);
         if (kVirial)
             for (int k = 0; k < 6; ++k) w_sum[k] += static_cast<double>(w[k]);
         if (live && sub == 0)
         {
-            T* f = a.forces + 3LL * row;
+            T* f = a.forces + 3LL * slot;
             f[0] = fx;
             f[1] = fy;
             f[2] = fz;
             if (kKick)
             {
 #pragma clang fp contract(off)
-                T* v = a.velocities + 3LL * row;
-                const T inverse_mass = a.inverse_masses[row];
+                T* v = a.velocities + 3LL * slot;
+                const T inverse_mass = a.inverse_masses[slot];
                 const T vx = v[0] + half_timestep * fx * inverse_mass;
                 const T vy = v[1] + half_timestep * fy * inverse_mass;
                 const T vz = v[2] + half_timestep * fz * inverse_mass;
                 v[0] = vx;
                 v[1] = vy;
                 v[2] = vz;
-                kinetic += 0.5 * static_cast<double>(a.masses[row]) *
+                kinetic += 0.5 * static_cast<double>(a.masses[slot]) *
                            (static_cast<double>(vx) * vx + static_cast<double>(vy) * vy +
                             static_cast<double>(vz) * vz);
             }
         }
     }
-    double* out = partials + 8LL * blockIdx.x;
-    const double e = block_sum(energy_sum, shared);
+    double* shared = sh->sums;
+    const double e = kEnergy ? block_sum(energy_sum, shared) : 0.0;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (kVirial)
         for (int k = 0; k < 6; ++k) v[k] = block_sum(w_sum[k], shared);
@@ -1194,9 +1570,45 @@ __global__ void __launch_bounds__(kBlock, (kForceWaves<T, Pot, kVirial, kMode>))
         for (int k = 0; k < 6; ++k) out[1 + k] = v[k];
         out[7] = k_sum;
     }
-    if (threadIdx.x < 8)
-        for (index_t slot = blockIdx.x + gridDim.x; slot < slots; slot += gridDim.x)
-            partials[8LL * slot + threadIdx.x] = 0.0;
+}
+
+// The rows of the list as slots, for readout: out[slot * stride + k] = the slot's k-th listed
+// neighbor (in the row's own order), from its brick's halo; the row's entries past its
+// neighbors, up to its count rounded up to 32, get -1. One thread per slot.
+__global__ void __launch_bounds__(kBlock) md_entries_kernel(
+    const index_t* __restrict__ bin_offsets, std::int64_t bins, index_t nx, index_t ny,
+    index_t nz, index_t bx, index_t by, index_t bz, const index_t* __restrict__ counts,
+    const std::uint16_t* __restrict__ index, std::int64_t stride,
+    const index_t* __restrict__ halo_slots, std::int64_t halo_capacity, index_t num_atoms,
+    index_t* __restrict__ out)
+{
+    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (slot >= num_atoms) return;
+    // the slot's sub-bin: the last whose first slot is at most `slot` (and which is not empty)
+    std::int64_t lo = 0, hi = bins - 1;
+    while (lo < hi)
+    {
+        const std::int64_t mid = (lo + hi + 1) / 2;
+        if (bin_offsets[mid] <= slot)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    index_t x, y, z;
+    bin_coordinates(lo, nx, ny, nz, &x, &y, &z);
+    const index_t dims[3] = {nx, ny, nz};
+    const index_t size[3] = {bx, by, bz};
+    const std::int64_t brick = brick_of(x, y, z, dims, size);
+    const index_t count = counts[slot] < stride ? counts[slot] : static_cast<index_t>(stride);
+    const index_t limit = (count + 31) & ~31;
+    index_t listed = 0;
+    for (index_t p = 0; p < limit; ++p)
+    {
+        const unsigned local = index[slot * stride + p];
+        if (local != 0 && local <= halo_capacity)
+            out[slot * stride + listed++] = halo_slots[brick * halo_capacity + local - 1];
+    }
+    while (listed < limit) out[slot * stride + listed++] = -1;
 }
 
 // One block of 1024 threads reduces the force kernel's partials.
@@ -1210,24 +1622,34 @@ __global__ void __launch_bounds__(kFinishThreads) finish_step_kernel(StepArgs<T>
                                                                      int blocks,
                                                                      int kinetic_from_partials)
 {
-    __shared__ double shared[16];
-    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // a block's 8 partials are 64 contiguous bytes: two 32-byte loads per block, all of a
-    // thread's loads in flight together (the grid is at most a few thousand blocks)
-    for (int b = threadIdx.x; b < blocks; b += kFinishThreads)
+    __shared__ double column[kFinishThreads];
+    // thread t sums partial (t % 8) of blocks t / 8, t / 8 + 128, ...: coalesced 8-byte reads,
+    // unrolled so that several are in flight, each sum in a fixed order
+    const int part = threadIdx.x % 8;
+    const std::int64_t values = 8LL * blocks;
+    double sum = 0.0;
+    std::int64_t i = threadIdx.x;
+    for (; i + 3 * kFinishThreads < values; i += 4 * kFinishThreads)
     {
-        const double4* p = reinterpret_cast<const double4*>(partials + 8LL * b);
-        const double4 lo = p[0], hi = p[1];
-        sums[0] += lo.x;
-        sums[1] += lo.y;
-        sums[2] += lo.z;
-        sums[3] += lo.w;
-        sums[4] += hi.x;
-        sums[5] += hi.y;
-        sums[6] += hi.z;
-        sums[7] += hi.w;
+        const double v0 = partials[i];
+        const double v1 = partials[i + kFinishThreads];
+        const double v2 = partials[i + 2 * kFinishThreads];
+        const double v3 = partials[i + 3 * kFinishThreads];
+        sum += v0;
+        sum += v1;
+        sum += v2;
+        sum += v3;
     }
-    for (int k = 0; k < 8; ++k) sums[k] = block_sum(sums[k], shared);
+    for (; i < values; i += kFinishThreads) sum += partials[i];
+    column[threadIdx.x] = sum;
+    __syncthreads();
+    // then partial k over the threads t % 8 == k, in order, by lane k of the first wave
+    if (threadIdx.x >= kWave) return;
+    double total = 0.0;
+    if (threadIdx.x < 8)
+        for (int t = part; t < kFinishThreads; t += 8) total += column[t];
+    double sums[8];
+    for (int k = 0; k < 8; ++k) sums[k] = read_lane(total, k);
     if (threadIdx.x != 0) return;
     if (a.energy) a.energy[0] = static_cast<T>(0.5 * sums[0]);
     if (a.virial)
@@ -1343,6 +1765,12 @@ StepArgs<T> step_args(const ndn_md_system& s, const ndn_md_settings* settings,
         a.inner_index = m.inner_index;
         a.prune_positions = static_cast<T*>(m.prune_positions);
     }
+    for (int k = 0; k < 3; ++k) a.brick[k] = m.brick[k];
+    a.bricks = m.bricks;
+    a.halo_capacity = m.halo_capacity;
+    a.halo_slots = m.halo_slots;
+    a.halo_runs = m.halo_runs;
+    a.halo_counts = m.halo_counts;
     return a;
 }
 
@@ -1358,8 +1786,7 @@ inline unsigned gated_grid(std::int64_t items, int per_block = kBlock)
 }
 
 template <typename T>
-int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream_t stream,
-                      bool prune = true)
+int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream_t stream)
 {
     const std::int64_t n = a.num_atoms;
     md_assign_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
@@ -1374,166 +1801,146 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     NDN_LAUNCHED("md_order");
     md_unpack_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_unpack");
+    md_halo_kernel<T><<<gated_grid(a.bricks, kBlock / kWave), kBlock, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_halo");
     const T r = static_cast<T>(radius);
     const T r2 = r * r;
     // culling keeps every sub-bin a pair within R could sit in, with room for rounding
     const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
-    md_fill_kernel<T><<<gated_grid(n, kBlock / kWave), kBlock, 0, stream>>>(a, r2, cull * cull,
-                                                                            forced);
+    md_fill_kernel<T><<<gated_grid(a.bins, kBlock / kWave), kBlock, 0, stream>>>(
+        a, r2, cull * cull, forced);
     NDN_LAUNCHED("md_fill");
-    if (prune && a.inner_index != nullptr && a.stride > 0)
-    {
-        const T inner = a.inner_radius;
-        md_prune_kernel<T><<<gated_grid(n, kBlock / 4), kBlock, 0, stream>>>(a, inner,
-                                                                             inner * inner, forced);
-        NDN_LAUNCHED("md_prune");
-    }
     return NDN_OK;
 }
 
-// Tiles of kBlock / G rows.
-std::int64_t force_tiles(std::int64_t num_atoms, int group)
+// Dynamic LDS of a brick's block: the halo (with the sentinel) and the block's bookkeeping.
+template <typename T>
+std::size_t brick_block_bytes(std::int64_t halo_capacity)
 {
-    const std::int64_t rows_per_tile = kBlock / group;
-    return (num_atoms + rows_per_tile - 1) / rows_per_tile;
+    return halo_bytes<T>(halo_capacity) + kBrickSharedBytes;
 }
 
-// The persistent grid: as many blocks as are resident on the device at once, a multiple of 8
-// (one share per XCD), and no more than the tiles need (so never more than xcd_grid(tiles)).
+
+// The LDS a brick block's halo may take: what one block may hold, and no more than half a CU's
+// LDS (two blocks, 32 waves of fp32, share a CU), less the block's bookkeeping.
+std::size_t max_halo_bytes()
+{
+    int device = 0, per_block = 65536, per_unit = 0;
+    if (hipGetDevice(&device) != hipSuccess) return 65536 - kBrickSharedBytes;
+    if (hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+        hipSuccess)
+        per_block = 65536;
+    if (hipDeviceGetAttribute(&per_unit, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              device) != hipSuccess)
+        per_unit = 0;
+    std::size_t bytes = static_cast<std::size_t>(std::max(per_block, 65536));
+    if (per_unit > 0) bytes = std::min(bytes, static_cast<std::size_t>(per_unit) / 2);
+    return bytes - kBrickSharedBytes;
+}
+
+// Kernels taking more than 64 KiB of dynamic LDS say so before their launch (a refusal shows
+// as the launch's own error).
 template <typename Kernel>
-unsigned persistent_grid(Kernel kernel, std::int64_t tiles)
+void allow_shared(Kernel kernel, std::size_t bytes)
 {
-    int device = 0, units = 0, per_unit = 0;
-    if (hipGetDevice(&device) != hipSuccess ||
-        hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device) !=
-            hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, kernel, kBlock, 0) != hipSuccess)
-        return xcd_grid(tiles);
-    const std::int64_t resident =
-        static_cast<std::int64_t>(std::max(units, 1)) * std::max(per_unit, 1);
-    return xcd_grid(std::min(resident, tiles));
+    if (bytes > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(bytes));
 }
 
+// The force launch of one schedule.
 template <typename T, class Pot, int G, bool kVirial, bool kKick>
-unsigned launch_force_kernel(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
-                             T half_timestep, double* partials, bool refresh_pair,
-                             hipStream_t stream)
+void launch_force_kernel(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
+                         T half_timestep, double* partials, unsigned grid, hipStream_t stream)
 {
-    const std::int64_t tiles = force_tiles(a.num_atoms, G);
-    if constexpr (kKick && G <= 8)
-    {
-        if (refresh_pair)
-        {
-            // a step with a pruned list: the two gated launches share one grid (and so one set
-            // of partial slots for the finish)
-            auto plain = force_kernel<T, Pot, G, kVirial, kKick, 1>;
-            auto refresh = force_kernel<T, Pot, G, kVirial, kKick, 2>;
-            const unsigned plain_grid = persistent_grid(plain, tiles);
-            const unsigned refresh_grid = persistent_grid(refresh, tiles);
-            const index_t slots = static_cast<index_t>(std::max(plain_grid, refresh_grid));
-            plain<<<plain_grid, kBlock, 0, stream>>>(a, pot, cutoff, cutoff_squared,
-                                                     half_timestep, partials,
-                                                     static_cast<index_t>(tiles), slots);
-            refresh<<<refresh_grid, kBlock, 0, stream>>>(a, pot, cutoff, cutoff_squared,
-                                                         half_timestep, partials,
-                                                         static_cast<index_t>(tiles), slots);
-            return static_cast<unsigned>(slots);
-        }
-    }
-    auto kernel = force_kernel<T, Pot, G, kVirial, kKick, 0>;
-    const unsigned grid = persistent_grid(kernel, tiles);
-    kernel<<<grid, kBlock, 0, stream>>>(a, pot, cutoff, cutoff_squared, half_timestep, partials,
-                                        static_cast<index_t>(tiles), static_cast<index_t>(grid));
-    return grid;
+    const std::size_t lds = brick_block_bytes<T>(a.halo_capacity);
+    allow_shared(force_kernel<T, Pot, G, kVirial, kKick>, lds);
+    force_kernel<T, Pot, G, kVirial, kKick><<<grid, kBrickThreads<T>, lds, stream>>>(
+        a, pot, cutoff, cutoff_squared, half_timestep, partials);
 }
 
 template <typename T, class Pot, int G>
-unsigned launch_force_g(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
-                        T half_timestep, bool virial, bool kick, double* partials,
-                        bool refresh_pair, hipStream_t stream)
+void launch_force_g(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
+                    T half_timestep, bool virial, bool kick, double* partials, unsigned grid,
+                    hipStream_t stream)
 {
     if (virial && kick)
-        return launch_force_kernel<T, Pot, G, true, true>(a, pot, cutoff, cutoff_squared,
-                                                          half_timestep, partials, refresh_pair,
-                                                          stream);
-    if (kick)
-        return launch_force_kernel<T, Pot, G, false, true>(a, pot, cutoff, cutoff_squared,
-                                                           half_timestep, partials, refresh_pair,
-                                                           stream);
-    // evaluations without the kick never refresh
-    if (virial)
-        return launch_force_kernel<T, Pot, G, true, false>(a, pot, cutoff, cutoff_squared,
-                                                           half_timestep, partials, false, stream);
-    return launch_force_kernel<T, Pot, G, false, false>(a, pot, cutoff, cutoff_squared,
-                                                        half_timestep, partials, false, stream);
+        launch_force_kernel<T, Pot, G, true, true>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                                   partials, grid, stream);
+    else if (kick)
+        launch_force_kernel<T, Pot, G, false, true>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                                    partials, grid, stream);
+    else if (virial)
+        launch_force_kernel<T, Pot, G, true, false>(a, pot, cutoff, cutoff_squared, half_timestep,
+                                                    partials, grid, stream);
+    else
+        launch_force_kernel<T, Pot, G, false, false>(a, pot, cutoff, cutoff_squared,
+                                                     half_timestep, partials, grid, stream);
 }
 
 template <typename T, class Pot>
-unsigned launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff,
-                      T cutoff_squared, T half_timestep, bool virial, bool kick,
-                      double* partials, bool refresh_pair, hipStream_t stream)
+void launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff, T cutoff_squared,
+                  T half_timestep, bool virial, bool kick, double* partials, unsigned grid,
+                  hipStream_t stream)
 {
     switch (group)
     {
-        case 2:
-            return launch_force_g<T, Pot, 2>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                             virial, kick, partials, refresh_pair, stream);
         case 4:
-            return launch_force_g<T, Pot, 4>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                             virial, kick, partials, refresh_pair, stream);
+            launch_force_g<T, Pot, 4>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
+                                      partials, grid, stream);
+            break;
         case 8:
-            return launch_force_g<T, Pot, 8>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                             virial, kick, partials, refresh_pair, stream);
+            launch_force_g<T, Pot, 8>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
+                                      partials, grid, stream);
+            break;
         default:
-            return launch_force_g<T, Pot, 16>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                              virial, kick, partials, refresh_pair, stream);
+            launch_force_g<T, Pot, 16>(a, pot, cutoff, cutoff_squared, half_timestep, virial,
+                                       kick, partials, grid, stream);
     }
 }
 
-// The widest lane group. A tile holds kBlock / G rows, so the widest group has the most tiles;
-// the persistent grid never exceeds xcd_grid(tiles), so xcd_grid(force_tiles(N, kMaxGroup))
-// partial slots suffice for every schedule.
-constexpr int kMaxGroup = 16;
-
 int valid_group(int group)
 {
-    return group == 2 || group == 4 || group == 8 || group == 16;
+    return group == 4 || group == 8 || group == 16;
+}
+
+// The force launches' grid: one block per brick, rounded up to a multiple of 8 (one share per
+// XCD); each block owns one partial slot.
+inline unsigned force_grid(std::int64_t bricks)
+{
+    return xcd_grid(std::max<std::int64_t>(bricks, 1));
 }
 
 template <typename T>
 int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coefficients, int group,
                   bool virial, bool kick, double* partials, std::int64_t partial_slots,
-                  hipStream_t stream, int* grid_out, bool refresh_pair = false)
+                  hipStream_t stream, int* grid_out)
 {
-    NDN_REQUIRE(valid_group(group), "group must be 2, 4, 8 or 16");
-    // pruned rows are padded for waves of at most 32 rows, i.e. at least 2 lanes per row and
-    // tiles of at least 32 rows: at most 8 lanes per row
-    NDN_REQUIRE(a.inner_index == nullptr || group <= 8,
-                "with a pruned list the force schedule takes at most 8 lanes per row");
-    NDN_REQUIRE(static_cast<std::int64_t>(xcd_grid(force_tiles(a.num_atoms, group))) <=
-                    partial_slots,
-                "the force launch may have more blocks than partial slots");
-    NDN_REQUIRE(a.stride >= 8 && a.stride % 4 == 0,
-                "the list stride must be a multiple of 4 and at least 8");
-    // gathers take 32-bit byte offsets into the float4 copy of N + 1 slots
-    NDN_REQUIRE((static_cast<std::int64_t>(a.num_atoms) + 1) * 4 * sizeof(T) < (1LL << 32),
-                "too many atoms for the force kernel's 32-bit gather offsets");
+    NDN_REQUIRE(valid_group(group), "group must be 4, 8 or 16");
+    const unsigned grid = force_grid(a.bricks);
+    NDN_REQUIRE(static_cast<std::int64_t>(grid) <= partial_slots,
+                "the force launch has more blocks than partial slots");
+    NDN_REQUIRE(a.stride >= 64 && a.stride % 64 == 0,
+                "the list stride must be a positive multiple of 64");
+    NDN_REQUIRE(a.halo_capacity >= 1 && halo_bytes<T>(a.halo_capacity) <= max_halo_bytes(),
+                "the halo capacity must be at least 1 and its positions fit in the LDS a force "
+                "block may take (ndn_md_halo_limit)");
     const T cutoff = static_cast<T>(pot->cutoff);
     const T cutoff_squared = static_cast<T>(pot->cutoff * pot->cutoff);
     const T half_timestep = static_cast<T>(0.5 * a.timestep);
     const int rows = pot->num_types * (pot->num_types + 1) / 2;
-    unsigned grid;
     if (pot->kind == NDN_LJ && pot->num_types == 1)
     {
         const OneTypeLJ<T> lj{static_cast<const LJRow<T>*>(coefficients)};
-        grid = launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                               partials, refresh_pair, stream);
+        launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                        partials, grid, stream);
     }
     else if (pot->kind == NDN_LJ)
     {
         const TypedLJ<T> lj{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
-        grid = launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                               partials, refresh_pair, stream);
+        launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                        partials, grid, stream);
     }
     else
     {
@@ -1542,11 +1949,25 @@ int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coeffici
             padded, static_cast<const T*>(pot->start),
             padded + static_cast<std::int64_t>(rows) * (pot->knots + 2), nullptr,
             pot->num_types, static_cast<int>(pot->knots)}};
-        grid = launch_force<T>(a, table, group, cutoff, cutoff_squared, half_timestep, virial,
-                               kick, partials, refresh_pair, stream);
+        launch_force<T>(a, table, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                        partials, grid, stream);
     }
     NDN_LAUNCHED("md_force");
     *grid_out = static_cast<int>(grid);
+    return NDN_OK;
+}
+
+// The prune kernel over every brick: gated on the refresh flag, or forced.
+template <typename T>
+int launch_prune(const StepArgs<T>& a, int forced, hipStream_t stream)
+{
+    if (a.inner_index == nullptr) return NDN_OK;
+    NDN_REQUIRE(a.halo_capacity >= 1 && halo_bytes<T>(a.halo_capacity) <= max_halo_bytes(),
+                "the halo capacity must be at least 1 and fit the LDS of a block");
+    const std::size_t lds = brick_block_bytes<T>(a.halo_capacity);
+    allow_shared(md_prune_kernel<T>, lds);
+    md_prune_kernel<T><<<force_grid(a.bricks), kBrickThreads<T>, lds, stream>>>(a, forced);
+    NDN_LAUNCHED("md_prune");
     return NDN_OK;
 }
 
@@ -1561,13 +1982,13 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     NDN_LAUNCHED("md_prologue");
     first_half_kernel<T><<<blocks_for(s.num_atoms), kBlock, 0, stream>>>(a, timestep, half_timestep);
     NDN_LAUNCHED("md_first_half");
-    // the step's force launches refresh the pruned list themselves
-    const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream, false);
+    const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream);
     if (rebuilt != NDN_OK) return rebuilt;
+    const int pruned = launch_prune<T>(a, 0, stream);
+    if (pruned != NDN_OK) return pruned;
     int blocks = 0;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, settings->virial != 0,
-                                        true, s.partials, s.partial_slots, stream, &blocks,
-                                        a.inner_index != nullptr);
+                                        true, s.partials, s.partial_slots, stream, &blocks);
     if (forces != NDN_OK) return forces;
     finish_step_kernel<T><<<1, kFinishThreads, 0, stream>>>(a, s.partials, blocks, 1);
     NDN_LAUNCHED("md_finish");
@@ -1576,13 +1997,18 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
 
 template <typename T>
 int md_evaluate_impl(const ndn_md_system& s, const ndn_potential* pot, const ndn_md_list& m,
-                     int virial, int kick, double timestep, hipStream_t stream)
+                     int virial, int kick, double timestep, hipStream_t stream, int refresh = 0)
 {
     StepArgs<T> a = step_args<T>(s, nullptr, nullptr, m);
     a.timestep = timestep;
     PotentialView<T> view;
     const int prepared = prepare_potential<T>(pot, s.coefficients, stream, &view);
     if (prepared != NDN_OK) return prepared;
+    if (refresh)
+    {
+        const int pruned = launch_prune<T>(a, 1, stream);
+        if (pruned != NDN_OK) return pruned;
+    }
     int blocks = 0;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, virial != 0, kick != 0,
                                         s.partials, s.partial_slots, stream, &blocks);
@@ -1610,11 +2036,24 @@ int check_system(const ndn_md_system* s, const ndn_md_list* m)
     NDN_REQUIRE(m->scan_temp_bytes >= scan_bytes(bins + 1) && m->scan_temp_bytes >= 1024,
                 "scan_temp is too small");
     NDN_REQUIRE(m->stride == 0 || m->index != nullptr, "a list with rows needs its index");
-    NDN_REQUIRE(m->stride >= 0 && m->stride * s->num_atoms <= (1LL << 40), "bad stride");
-    NDN_REQUIRE(valid_group(m->group), "group must be 2, 4, 8 or 16");
-    NDN_REQUIRE(s->partial_slots >=
-                    static_cast<std::int64_t>(xcd_grid(force_tiles(s->num_atoms, m->group))),
-                "too few partial slots for this force schedule");
+    NDN_REQUIRE(m->stride >= 0 && m->stride % 64 == 0 && m->stride * s->num_atoms <= (1LL << 40),
+                "the stride must be a nonnegative multiple of 64");
+    NDN_REQUIRE(valid_group(m->group), "group must be 4, 8 or 16");
+    for (int k = 0; k < 3; ++k) NDN_REQUIRE(m->brick[k] >= 1, "brick sizes must be positive");
+    NDN_REQUIRE(kLayerBlock % m->brick[2] == 0, "brick[2] must divide 8");
+    NDN_REQUIRE(m->brick[1] * m->brick[2] <= kMaxBrickRuns &&
+                    (m->brick[1] + 4) * (m->brick[2] + 4) <= kMaxHaloRuns,
+                "bricks too large: brick[1] * brick[2] <= 64 and "
+                "(brick[1] + 4) * (brick[2] + 4) <= 128");
+    std::int64_t bricks = 1;
+    for (int k = 0; k < 3; ++k) bricks *= (m->dims[k] + m->brick[k] - 1) / m->brick[k];
+    NDN_REQUIRE(m->bricks == bricks, "bricks must be the product of ceil(dims / brick)");
+    NDN_REQUIRE(m->halo_slots && m->halo_runs && m->halo_counts, "a halo buffer is null");
+    NDN_REQUIRE(m->halo_capacity >= 0 && m->halo_capacity < 65535 &&
+                    m->halo_capacity * bricks <= (1LL << 40),
+                "the halo capacity must be below 65535");
+    NDN_REQUIRE(s->partial_slots >= static_cast<std::int64_t>(force_grid(bricks)),
+                "too few partial slots: one per brick, rounded up to a multiple of 8");
     return NDN_OK;
 }
 
@@ -1631,10 +2070,17 @@ int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     NDN_REQUIRE(num_atoms >= 1 && partial_slots && coefficient_bytes_out, "bad arguments");
-    // the most blocks any schedule may launch: the widest lane group's tile count
-    *partial_slots = std::max<std::int64_t>(xcd_grid(force_tiles(num_atoms, kMaxGroup)), 8);
+    // at most one brick per atom holds atoms; the list may ask for fewer (one per brick,
+    // rounded up to a multiple of 8)
+    *partial_slots = std::max<std::int64_t>(xcd_grid(num_atoms), 8);
     *coefficient_bytes_out = coefficient_bytes(pot, dtype == NDN_F32 ? 4 : 8);
     return NDN_OK;
+}
+
+int64_t ndn_md_halo_limit(int dtype)
+{
+    const std::size_t point = dtype == NDN_F32 ? sizeof(float4) : sizeof(double4);
+    return static_cast<int64_t>(max_halo_bytes() / point) - 1;
 }
 
 int64_t ndn_md_scratch_bytes(int dtype, int64_t num_atoms)
@@ -1696,6 +2142,43 @@ int ndn_md_evaluate(int dtype, const ndn_md_system* system, const ndn_potential*
         return md_evaluate_impl<float>(*system, pot, *md_list, virial, kick, timestep, stream);
     NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     return md_evaluate_impl<double>(*system, pot, *md_list, virial, kick, timestep, stream);
+}
+
+int ndn_md_refresh(int dtype, const ndn_md_system* system, const ndn_potential* pot,
+                   const ndn_md_list* md_list, ndn_stream_t stream)
+{
+    int checked = check_system(system, md_list);
+    if (checked != NDN_OK) return checked;
+    checked = check_potential(pot, system->num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
+    NDN_REQUIRE(md_list->inner_index && md_list->inner_counts && md_list->prune_positions,
+                "the list has no pruned list");
+    NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
+                "several atom types need the slot-ordered types");
+    if (dtype == NDN_F32)
+        return md_evaluate_impl<float>(*system, pot, *md_list, 0, 0, 0.0, stream, 1);
+    NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    return md_evaluate_impl<double>(*system, pot, *md_list, 0, 0, 0.0, stream, 1);
+}
+
+int ndn_md_list_entries(const ndn_md_list* md_list, int64_t num_atoms, int pruned, int32_t* out,
+                        ndn_stream_t stream)
+{
+    NDN_REQUIRE(md_list && out && num_atoms >= 1 && num_atoms <= INT32_MAX, "bad arguments");
+    NDN_REQUIRE(md_list->stride > 0 && md_list->halo_capacity > 0, "the list has no rows yet");
+    const std::uint16_t* index = pruned ? md_list->inner_index : md_list->index;
+    const int32_t* counts = pruned ? md_list->inner_counts : md_list->counts;
+    NDN_REQUIRE(index && counts && md_list->bin_offsets && md_list->halo_slots,
+                "a list buffer is null");
+    const std::int64_t bins =
+        static_cast<std::int64_t>(md_list->dims[0]) * md_list->dims[1] * md_list->dims[2];
+    md_entries_kernel<<<blocks_for(num_atoms), kBlock, 0, stream>>>(
+        md_list->bin_offsets, bins, md_list->dims[0], md_list->dims[1], md_list->dims[2],
+        md_list->brick[0], md_list->brick[1], md_list->brick[2], counts, index, md_list->stride,
+        md_list->halo_slots, md_list->halo_capacity, static_cast<index_t>(num_atoms), out);
+    NDN_LAUNCHED("md_list_entries");
+    return NDN_OK;
 }
 
 int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* pot,

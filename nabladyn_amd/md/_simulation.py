@@ -85,13 +85,38 @@ class MDList(_native.ctypes.Structure):
         ("prepare_total", _native.ctypes.c_void_p),
         ("inner_radius", _native.ctypes.c_double), ("inner_counts", _native.ctypes.c_void_p),
         ("inner_index", _native.ctypes.c_void_p), ("prune_positions", _native.ctypes.c_void_p),
+        ("brick", _native.ctypes.c_int32 * 3), ("bricks", _native.ctypes.c_int64),
+        ("halo_capacity", _native.ctypes.c_int64), ("halo_slots", _native.ctypes.c_void_p),
+        ("halo_runs", _native.ctypes.c_void_p), ("halo_counts", _native.ctypes.c_void_p),
     ]
 
 
-_GROUPS = (2, 4, 8, 16)
-# with a pruned list the rows are padded for schedules of at most 8 lanes per row
-_PRUNED_GROUPS = (2, 4, 8)
-_INVALID_CELL, _NON_FINITE, _OVERFLOW, _FULL_OVERFLOW = 1, 2, 8, 16
+_GROUPS = (4, 8, 16)
+_INVALID_CELL, _NON_FINITE, _OVERFLOW, _FULL_OVERFLOW, _HALO_OVERFLOW = 1, 2, 8, 16, 32
+_HALO_RUNS = 128  # halo run slots per brick
+# brick shapes (sub-bins along x, y, z), largest first: the first whose expected halo fits wins
+_BRICKS = ((4, 4, 8), (4, 2, 8), (2, 2, 8), (2, 2, 4), (2, 1, 4), (1, 1, 4), (1, 1, 2),
+           (1, 1, 1))
+
+
+def brick_shapes(dims: list[int], num_atoms: int, dtype: torch.dtype):
+    """The brick shapes worth trying for this grid, largest first: those whose expected halo
+    (atoms per sub-bin times its sub-bins, with a tenth to spare) fits the LDS budget."""
+    per_bin = num_atoms / math.prod(dims)
+    limit = halo_limit(dtype)
+    shapes = []
+    for shape in _BRICKS:
+        halo = math.prod(min(b + 4, d) for b, d in zip(shape, dims))
+        if 1.1 * halo * per_bin <= limit:
+            shapes.append(shape)
+    return shapes or [_BRICKS[-1]]
+
+
+def halo_limit(dtype: torch.dtype) -> int:
+    """The largest halo the force kernel's LDS holds on the current device."""
+    query = _native.lib().ndn_md_halo_limit
+    query.restype = _native.ctypes.c_int64
+    return int(query(_native.dtype_tag(dtype)))
 
 
 def sub_bin_dims(lengths: list[float], radius: float, dtype: torch.dtype) -> list[int]:
@@ -137,10 +162,10 @@ class _MDNeighbors:
             "unordered": torch.zeros(n, **ints),
             "order": torch.zeros(n, **ints),
             "counts": torch.zeros(n, **ints),
-            "index": torch.zeros(1, **ints),
+            "index": torch.zeros(1, dtype=torch.int16, device=device),
             "ids": torch.arange(n, **ints),
             "types": types.clone() if types is not None else None,
-            # row n: the sentinel that pads the list's rows (written by every rebuild)
+            # x, y, z, type of each slot (row n spare)
             "positions4": torch.zeros((n + 1, 4), dtype=dtype, device=device),
             "scratch": torch.zeros(int(scratch(_native.dtype_tag(dtype), _native.ctypes.c_int64(n))),
                                    dtype=torch.uint8, device=device),
@@ -153,7 +178,7 @@ class _MDNeighbors:
         if self.inner_skin is not None:
             b_ = self.buffers
             b_["inner_counts"] = torch.zeros(n, **ints)
-            b_["inner_index"] = torch.zeros(1, **ints)
+            b_["inner_index"] = torch.zeros(1, dtype=torch.int16, device=device)
             b_["prune_positions"] = torch.zeros((n, 3), dtype=dtype, device=device)
         b = self.buffers
         self.native = MDList(
@@ -174,17 +199,55 @@ class _MDNeighbors:
             rebuild_count=_native.ptr(self.rebuild_count),
         )
         self.dims = dims
+        self.dtype = dtype
+        self.set_bricks(brick_shapes(dims, n, dtype)[0])
 
     @property
     def radius(self) -> float:
         return self.cutoff + self.skin
 
+    def set_bricks(self, shape) -> None:
+        """Bricks of `shape` sub-bins; their halos unsized until `size_halo`."""
+        device = self.buffers["counts"].device
+        bricks = math.prod(-(-d // b) for d, b in zip(self.dims, shape))
+        self.brick = tuple(shape)
+        self.bricks = bricks
+        self.buffers["halo_runs"] = torch.zeros(bricks * _HALO_RUNS, dtype=torch.int32, device=device)
+        self.buffers["halo_counts"] = torch.zeros(bricks, dtype=torch.int32, device=device)
+        self.buffers["halo_slots"] = torch.zeros(1, dtype=torch.int32, device=device)
+        self.native.brick = (_native.ctypes.c_int32 * 3)(*shape)
+        self.native.bricks = bricks
+        self.native.halo_capacity = 0
+        for name in ("halo_runs", "halo_counts", "halo_slots"):
+            setattr(self.native, name, _native.ptr(self.buffers[name]))
+
+    @property
+    def halo_needed(self) -> int:
+        """The largest halo of the last build."""
+        return int(self.buffers["scan_temp"][656:660].view(torch.int32).item())
+
+    def size_halo(self, needed: int) -> None:
+        """Room for halos of `needed` atoms with a margin (half the rows' multiplier's), up to
+        what LDS holds."""
+        margin = 1.0 + 0.5 * (self.multiplier - 1.0)
+        capacity = min(max(1, math.ceil(margin * needed)), halo_limit(self.dtype))
+        self.buffers["halo_slots"] = torch.zeros(self.bricks * capacity, dtype=torch.int32,
+                                                 device=self.buffers["counts"].device)
+        self.native.halo_slots = _native.ptr(self.buffers["halo_slots"])
+        self.native.halo_capacity = capacity
+
+    @property
+    def halo_capacity(self) -> int:
+        return int(self.native.halo_capacity)
+
     def size_rows(self, max_count: int) -> None:
         """Allocate the rows for `max_count` neighbors times the capacity multiplier."""
-        stride = max(8, math.ceil(self.multiplier * max_count))
-        stride = (stride + 7) // 8 * 8
+        # a multiple of 64: a row's last block of 4G entries (G <= 16 lanes per row) stays in it
+        stride = max(64, math.ceil(self.multiplier * max_count))
+        stride = (stride + 63) // 64 * 64
         n = self.buffers["counts"].numel()
-        self.buffers["index"] = torch.zeros(n * stride, dtype=torch.int32,
+        # 16-bit local indices into the brick's halo
+        self.buffers["index"] = torch.zeros(n * stride, dtype=torch.int16,
                                             device=self.buffers["counts"].device)
         if self.inner_skin is not None:
             # a pruned row is a subset of its full row: the same stride always holds it
@@ -228,6 +291,11 @@ class _MDNeighbors:
                     f"a neighbor row needs {needed} entries but holds {self.stride}; "
                     "raise capacity_multiplier",
                 )
+            if flags & _HALO_OVERFLOW:
+                causes.append(
+                    f"a brick's halo needs {self.halo_needed} atoms but holds "
+                    f"{self.halo_capacity}; raise capacity_multiplier",
+                )
             raise ValueError("; ".join(causes))
 
     @property
@@ -241,12 +309,17 @@ class _MDNeighbors:
         b = self.buffers
         counts = b["inner_counts" if pruned else "counts"].cpu()
         stride = self.stride
-        index = b["inner_index" if pruned else "index"].view(-1, stride).cpu()
-        ids = b["ids"].cpu()
-        rows = torch.repeat_interleave(torch.arange(counts.numel()), counts)
-        columns = torch.cat([torch.arange(int(c)) for c in counts]) if counts.numel() else counts
-        neighbors = index[rows, columns].long()
-        return torch.stack([ids[rows].long(), ids[neighbors]], dim=1)
+        slots = torch.full((counts.numel() * stride,), -1, dtype=torch.int32,
+                           device=b["counts"].device)
+        _native.call("md_list_entries", _native.ctypes.byref(self.native),
+                     _native.ctypes.c_int64(counts.numel()), 1 if pruned else 0,
+                     _native.ptr(slots), _native.stream(slots.device))
+        # a row's slots, then -1 past its neighbors (a pruned row's count includes the
+        # sentinels that even out its lanes' shares)
+        index = slots.view(-1, stride).cpu().long()
+        ids = b["ids"].cpu().long()
+        rows, columns = torch.nonzero(index >= 0, as_tuple=True)
+        return torch.stack([ids[rows], ids[index[rows, columns]]], dim=1)
 
 
 def _positive(name: str, value) -> float:
@@ -369,8 +442,8 @@ class Simulation:
         _native.call("md_workspace", _native.dtype_tag(dtype), _native.ctypes.byref(pot),
                      _native.ctypes.c_int64(num_atoms), _native.ctypes.byref(slots),
                      _native.ctypes.byref(coef))
-        self.partials = torch.zeros(slots.value * 8, **wide)
         self.coefficients = torch.zeros(max(coef.value, 256), dtype=torch.uint8, device=device)
+        self.partials = torch.zeros(8 * 8, **wide)
         self._system = MDSystem(
             num_atoms=num_atoms, positions=_native.ptr(self.positions),
             velocities=_native.ptr(self.velocities), forces=_native.ptr(self.forces),
@@ -379,35 +452,64 @@ class Simulation:
             energy=_native.ptr(self.energy.view(1)), virial=_native.ptr(self.virial),
             thermostat=_native.ptr(self.thermostat), chain_masses=_native.ptr(self.chain_masses),
             kinetic=_native.ptr(self.kinetic), partials=_native.ptr(self.partials),
-            partial_slots=slots.value, coefficients=_native.ptr(self.coefficients),
+            partial_slots=8, coefficients=_native.ptr(self.coefficients),
             coefficient_bytes=self.coefficients.numel(),
         )
         self._settings = MDSettings(
             timestep=timestep, kT=self._kT, degrees_of_freedom=self._degrees_of_freedom,
             thermostat=1 if self._nvt else 0, virial=1 if self._virial_every_step else 0,
         )
-        # the first build: sort and count, size the rows, then fill them
-        self._rebuild(forced=True)
-        self._neighbors._check_status()
-        self._neighbors.size_rows(int(self._neighbors.buffers["counts"].max()))
-        self._rebuild(forced=True)
-        self._neighbors._check_status()
-        self._neighbors.rebuild_count.fill_(1)
-        self._groups = _PRUNED_GROUPS if self._neighbors.inner_skin is not None else _GROUPS
-        if force_group is None:
-            self.group = self._tune_group()
-        else:
+        self._groups = _GROUPS
+        if force_group is not None and force_group not in self._groups:
+            raise ValueError(f"force_group must be one of {self._groups}")
+        if force_group is not None:
             # a fixed schedule: the force sums' order, hence every bit, independent of timing
-            if force_group not in self._groups:
-                raise ValueError(f"force_group must be one of {self._groups}")
             self._neighbors.native.group = force_group
-            self.group = force_group
+        self._first_build()
+        if self._neighbors.inner_skin is not None:
+            _native.call("md_refresh", _native.dtype_tag(dtype), _native.ctypes.byref(self._system),
+                         _native.ctypes.byref(pot), _native.ctypes.byref(self._neighbors.native),
+                         _native.stream(device))
+        self.group = self._tune_group() if force_group is None else force_group
         self._evaluate(virial=True)
         self.kinetic.copy_(0.5 * (self.slot_masses.double()[:, None]
                                   * self.velocities.double().square()).sum())
         self._failure: str | None = None
         self._graph = self._capture()
         self._chunk_graph = self._capture(_CHUNK_STEPS)
+
+    def _first_build(self) -> None:
+        """Sort and count (rows and bricks' halos), size them, then fill. Bricks shrink when the
+        largest halo would not fit in LDS."""
+        nb = self._neighbors
+        shapes = brick_shapes(nb.dims, self._num_atoms, self._dtype)
+        margin = 1.0 + 0.5 * (nb.multiplier - 1.0)
+        for k, shape in enumerate(shapes):
+            nb.set_bricks(shape)
+            self._set_partials(nb.bricks)
+            nb.buffers["scan_temp"][656:660].zero_()
+            self._rebuild(forced=True)
+            nb._check_status()
+            needed = nb.halo_needed
+            if margin * needed <= halo_limit(self._dtype) or k == len(shapes) - 1:
+                break
+        if needed > halo_limit(self._dtype):
+            raise ValueError(
+                f"a brick's halo holds {needed} atoms, more than the {halo_limit(self._dtype)} "
+                "the force kernel's LDS takes (the system is too dense for its sub-bins)")
+        nb.size_rows(int(nb.buffers["counts"].max()))
+        nb.size_halo(needed)
+        self._rebuild(forced=True)
+        nb._check_status()
+        nb.rebuild_count.fill_(1)
+
+    def _set_partials(self, bricks: int) -> None:
+        """One fp64 partial slot (8 sums) per force block: a block per brick, rounded up to a
+        multiple of 8."""
+        slots = max(8, -(-bricks // 8) * 8)
+        self.partials = torch.zeros(slots * 8, dtype=torch.float64, device=self._device)
+        self._system.partials = _native.ptr(self.partials)
+        self._system.partial_slots = slots
 
     # -- the step ---------------------------------------------------------------------------
 
@@ -434,8 +536,8 @@ class Simulation:
         )
 
     def _tune_group(self, launches: int = 3) -> int:
-        """The autotuner of the force schedule: lanes per neighbor row (2 to 16, at most 8 with a
-        pruned list), each timed with HIP events on this frame; the fastest wins. Syncs."""
+        """The autotuner of the force schedule: lanes per neighbor row (4, 8 or 16), each timed
+        with HIP events on this frame; the fastest wins. Syncs."""
         best, best_time = 8, math.inf
         for group in self._groups:
             self._neighbors.native.group = group
