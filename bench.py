@@ -214,8 +214,9 @@ def main() -> int:
     parser.add_argument(
         "--kernel", choices=("force", "copy"),
         help="only launch one kernel eagerly, for counter collection: 'force' builds the "
-             "2M-atom system and launches its force kernel 20 times; 'copy' copies 1 GiB on the "
-             "device 20 times (the scale of the traffic counters)")
+             "2M-atom system, runs 100 steps, then launches its force kernel 23 times (the last "
+             "23 dispatches of that kernel); 'copy' copies 1 GiB on the device 20 times (the "
+             "scale of the traffic counters)")
     parser.add_argument("--no-serving", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -243,10 +244,14 @@ def main() -> int:
     log("building the 2,097,152-atom system ...")
     sim, x, cell = build_simulation(torch)
     if args.kernel == "force":
+        # the liquid, not the first frame's lattice: 100 steps first
+        sim.run(100)
+        torch.cuda.synchronize()
         seconds = time_force_kernel(torch, sim, launches=20)
         pairs = sim._neighbors.num_pairs
-        print(json.dumps({"kernel": "force", "launches": 23, "ms": seconds * 1e3,
-                          "pairs_within_radius": pairs, "group": sim.group}))
+        print(json.dumps({"kernel": "force", "launches": 23, "steps_before": 100,
+                          "ms": seconds * 1e3, "pairs_within_radius": pairs,
+                          "group": sim.group}))
         return 0
     n = x.shape[0]
     torch.cuda.synchronize()
@@ -316,7 +321,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "force_kernel (full-list LJ forces + second half kick)",
+            "kernel": "force_kernel (one block per brick, halo in LDS, pruned rows; LJ forces + second half kick)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

@@ -4,21 +4,25 @@
 // The resident frame is kept in cell order ("slots"): at every rebuild the atoms are counting-
 // sorted into sub-bins at least half the list radius wide, ordered by atom id inside a sub-bin
 // (so every run is deterministic), and the whole frame is permuted with them. Every per-atom
-// access of a step is then coalesced, and a neighbor's position is a short gather from nearby
-// memory. The frame is read out in atom order only on demand.
+// access of a step is then coalesced. The frame is read out in atom order only on demand.
+//
+// The list works in bricks (boxes of sub-bins): a brick's halo (every sub-bin within 2 of it) is
+// listed once per build, and a row names its neighbors by 16-bit indices into its brick's halo.
 //
 //   prologue     one block: the chain's opening half step (fp64), the reuse rule's preparation
 //                (update/update.cuh:27-75), the potential's coefficient rows
-//   first_half   per slot: chain scale, half kick, drift, wrap, image count, the float4 copy the
-//                force kernel gathers from, and the squared displacement since the last rebuild
-//                reduced to the rebuild flag (update/update.cuh:79-116)
-//   rebuild      gated on that flag: sub-bin counting sort, permutation of the frame, then one
-//                wave per atom tests the candidates of its culled 5 x 5 x 5 sub-bin stencil 64 at
-//                a time and ballots the hits into its row of a fixed-stride full list
+//   first_half   per slot: chain scale, half kick, drift, wrap, image count, the float4 copy of
+//                the positions, and the squared displacements since the last build and refresh
+//                reduced to the rebuild and refresh flags (update/update.cuh:79-116)
+//   rebuild      gated on the rebuild flag: sub-bin counting sort, permutation of the frame, the
+//                bricks' halos, then one wave per sub-bin tests the candidates of its culled
+//                5 x 5 x 5 stencil 64 at a time and ballots the hits into its row of a
+//                fixed-stride full list
 //   prune        gated on a build, or on an atom having moved half the pruned list's skin since
 //                its last refresh: keeps the full rows' pairs within cutoff + that skin, so the
 //                force kernel reads about two thirds of the entries
-//   force        a lane group per row reads its pruned row coalesced, sums the force in
+//   force        one block per brick stages the brick's halo in LDS; a lane group per row reads
+//                its pruned row and the neighbors' positions from LDS, sums the force in
 //                registers, then the second half kick and the partial sums of E, W and K
 //   finish       one block: E, W, K in fixed order and the chain's closing half step; its
 //                velocity factor is applied by the next first_half (or by settle)
@@ -102,7 +106,7 @@ struct Vec4<double>
     using type = double4;
 };
 
-// The sentinel that pads the rows of the MD list: slot N of the float4 copy holds
+// The sentinel that pads the rows of the MD list: entry 0 of every brick's halo holds
 // (kFarAway, kFarAway, kFarAway), far beyond any cutoff from every atom, yet its squared
 // distances stay finite in fp32.
 template <typename T>
@@ -632,11 +636,6 @@ __global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int fo
         {
             for (int k = 0; k < 3; ++k) a.reference_cell[k] = a.cell[k];
             if (a.rebuild_count) ++a.rebuild_count[0];
-            // the sentinel that pads the list's rows
-            V4 far;
-            far.x = far.y = far.z = kFarAway<T>;
-            far.w = T(0);
-            reinterpret_cast<V4*>(a.positions4)[a.num_atoms] = far;
         }
     }
 }
