@@ -188,9 +188,8 @@ typedef struct ndn_list_state {
  * divides 8; brick[1] * brick[2] <= 64; (brick[1] + 4) * (brick[2] + 4) <= 128); a brick's halo
  * is every sub-bin within 2 of it (periodic), listed once per build in halo_slots, and a row
  * holds 16-bit local indices into its brick's halo: 0 the sentinel (a point far from every
- * atom) that pads the row, k >= 1 the slot halo_slots[brick * halo_capacity + k - 1]. Within
- * each block of 32 entries, sorted entry j sits at (j % 8) * 4 + j / 8 (a lane's 4 entries
- * together). stride is a multiple of 64. `group` lanes of a wave read one row in the force
+ * atom) that pads the row up to its count rounded up to 32, k >= 1 the slot
+ * halo_slots[brick * halo_capacity + k - 1], in no set order. stride is a multiple of 64. `group` lanes of a wave read one row in the force
  * kernel (4, 8 or 16). status [2] latches errors as the neighbor list's does, with
  * NDN_STATUS_FULL_OVERFLOW when a row needs more than `stride` entries (status[1] = the width
  * needed) and NDN_STATUS_HALO_OVERFLOW when a halo needs more than halo_capacity. scan_temp
@@ -255,8 +254,9 @@ int ndn_md_rebuild(int dtype, const ndn_md_system* system, const ndn_list_state*
 
 /* One MD step (NVE or NVT), launched on the stream with no sync or allocation, so a HIP graph
  * can capture it: chain half step, first half kick + drift + wrap, the reuse check and the gated
- * rebuild, forces over the full list with the second half kick and K, then E and the chain's
- * closing half step. Replaces one replay of the reference's captured step
+ * rebuild, the gated refresh of the pruned list, forces over the pruned list with the second
+ * half kick and K, then the chain's closing half step (E is not summed: `energy` keeps the last
+ * evaluation's). Replaces one replay of the reference's captured step
  * (md/_simulation.py:316-364: first_half, NeighborList._update_wrapped, second_half, closing). */
 int ndn_md_step(int dtype, const ndn_md_system* system, const ndn_md_settings* settings,
                 const ndn_potential* pot, const ndn_list_state* list, const ndn_md_list* md_list,

@@ -783,14 +783,6 @@ __device__ __forceinline__ Halo brick_halo(const BrickBox& box, const index_t di
     return h;
 }
 
-// Position of sorted entry k within its row: blocks of 32, in which lane s of 8 holds entries
-// s, s + 8, s + 16 and s + 24 in 4 consecutive places, so that one 8-byte read gives a lane 4
-// entries and the 8 lanes of a row, taking their u-th entries together, read 8 consecutive ones.
-__host__ __device__ __forceinline__ index_t entry_position(index_t k)
-{
-    return (k & ~31) | ((k & 7) << 2) | ((k >> 3) & 3);
-}
-
 // One wave per brick: the runs of its halo, their local starts (halo_runs, 1-based), the slots
 // in local order (halo_slots) and the count. With halo_capacity 0 it only counts. The largest
 // count goes to the header (byte 656) for sizing; a count above the capacity sets
@@ -862,22 +854,87 @@ __global__ void __launch_bounds__(kBlock) md_halo_kernel(StepArgs<T> a, int forc
     }
 }
 
+// The periodic frame of a brick. Along each axis every position p is replaced by its image nearest
+// the brick, written so that a plain difference of two images is bit for bit the reference's
+// minimum-image displacement (space.py:11-19) of every pair within `reach` of the brick's atoms:
+// a brick in the lower half keeps p, or takes p - L for an atom more than L / 2 above its center
+// (then p > L / 2, so p - L is exact, and c - (p - L) is the reference's c - (o - L)); a brick
+// whose halo passes the upper face moves its whole frame down by L: p - L for every atom but
+// those more than L / 2 below its center, which keep p (p - L is exact for all atoms within
+// reach, which lie in the upper half, and (c - L) - p is the reference's (c - L) - o). The frame
+// exists only when every atom within reach of the brick lies within L / 2 of its center (and, for
+// the moved frame, in the upper half); otherwise `exact` is false and the walk takes the
+// reference's displacement per pair on unmoved positions.
+template <typename T>
+struct BrickFrame
+{
+    T center[3], length[3];
+    bool moved[3];
+    bool exact;
+
+    __device__ __forceinline__ T image(T p, int axis) const
+    {
+        const T offset = p - center[axis];
+        const T half = T(0.5) * length[axis];
+        if (moved[axis]) return offset < -half ? p : p - length[axis];
+        return offset > half ? p - length[axis] : p;
+    }
+
+    template <typename V4>
+    __device__ __forceinline__ V4 place(V4 p) const
+    {
+        if (!exact) return p;
+        p.x = image(p.x, 0);
+        p.y = image(p.y, 1);
+        p.z = image(p.z, 2);
+        return p;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const BrickBox& box,
+                                                     T reach)
+{
+    BrickFrame<T> f;
+    f.exact = true;
+    const T widths[3] = {a.wx, a.wy, a.wz};
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        const T w = widths[axis];
+        const T lo = static_cast<T>(box.lo[axis]) * w;
+        const T hi = static_cast<T>(box.hi[axis]) * w;
+        const T length = a.cell[axis];
+        f.center[axis] = T(0.5) * (lo + hi);
+        f.length[axis] = length;
+        // the brick's atoms may have left its sub-bins by up to half the list's skin since the
+        // build; a sub-bin of margin (at least half the list's radius) covers that and the
+        // rounding of the sub-bin bounds
+        const T extent = T(0.5) * (hi - lo) + reach + w;
+        f.moved[axis] = f.center[axis] + extent > length;
+        const bool inside = extent < T(0.5) * length;
+        const bool upper = !f.moved[axis] || f.center[axis] - extent > T(0.5) * length;
+        f.exact = f.exact && inside && upper;
+    }
+    return f;
+}
+
 // One wave per sub-bin. Its atoms (the centers) share one stencil: lanes 0..49 hold the slot
 // ranges of the 25 stencil rows (runs of x-consecutive sub-bins; two parts where a run wraps
 // around x), culled by the distance between sub-bins, so every sub-bin a pair within R could
 // touch is kept, and the offset from a slot of the range to its local index in the brick's halo
 // (each range lies inside one run of the halo). The candidates are packed 64 at a time; each
 // chunk is gathered once and tested against every center in turn with the exact minimum-image
-// expression, and each center's hits are ballot-compacted into its row (at entry_position).
+// expression, and each center's hits are ballot-compacted into its row, in the order met.
 // Rows are then padded with the sentinel 0 up to their count rounded up to 32, so the force kernel
 // reads whole blocks unmasked. With stride 0 it only counts (the first frame, to size the rows).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius_squared,
+__global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius,
                                                          T cull_squared, int forced)
 {
 #pragma clang fp contract(off)
     using V4 = typename Vec4<T>::type;
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    const T radius_squared = radius * radius;
     const int lane = lane_id();
     const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
@@ -888,18 +945,24 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
     const int span_y = full_y ? ny : kStencil;
     const int span_z = full_z ? nz : kStencil;
     const bool writing = a.stride > 0;
+    const index_t stride = static_cast<index_t>(a.stride);
     // one wave per sub-bin, the waves of the grid striding over the sub-bins
     const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (kBlock / kWave);
     for (std::int64_t bin =
              static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
          bin < a.bins; bin += waves)
     {
-        const index_t begin = a.bin_offsets[bin];
-        const index_t end = a.bin_offsets[bin + 1];
+        // the sub-bin is the wave's: its bounds, and every center's row address below, scalar
+        const index_t begin = __builtin_amdgcn_readfirstlane(a.bin_offsets[bin]);
+        const index_t end = __builtin_amdgcn_readfirstlane(a.bin_offsets[bin + 1]);
         if (begin == end) continue;
         index_t hx, hy, hz;
         bin_coordinates(bin, nx, ny, nz, &hx, &hy, &hz);
         const index_t brick = brick_of(hx, hy, hz, dims, a.brick);
+        // the sub-bin's own frame: the plain difference of placed positions is the reference's
+        // minimum image for every pair within R of its atoms
+        const BrickBox own{{hx, hy, hz}, {hx + 1, hy + 1, hz + 1}};
+        const BrickFrame<T> frame = brick_frame(a, own, radius);
         const Halo h = brick_halo(brick_box(brick, dims, a.brick), dims);
         index_t a0, a1, b0, b1;
         x_parts(h.xs, h.sx, nx, &a0, &a1, &b0, &b1);
@@ -974,14 +1037,13 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
         const index_t inclusive = wave_inclusive_sum(length);
         const index_t start = inclusive - length;
         const index_t total = __shfl(inclusive, kWave - 1, kWave);
-        // near a face (or with a short axis) the images come from the minimum image per pair
-        const bool wraps = full_x || full_y || full_z || hx < 2 || hx + 2 >= nx || hy < 2 ||
-                           hy + 2 >= ny || hz < 2 || hz + 2 >= nz;
+        // without a frame (a short axis) the images come from the minimum image per pair
+        const bool wraps = !frame.exact;
         // the centers, 64 at a time: lane k holds center first + k and its count
         for (index_t first = begin; first < end; first += kWave)
         {
             const int centers = end - first < kWave ? end - first : kWave;
-            const V4 mine = pos[lane < centers ? first + lane : first];
+            const V4 mine = frame.place(pos[lane < centers ? first + lane : first]);
             index_t mine_count = 0;
             for (index_t chunk = 0; chunk < total; chunk += kWave)
             {
@@ -999,7 +1061,7 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
                 index_t local = slot + __shfl(to_local, r, kWave);
                 // past an overflowing halo's capacity: the sentinel (the status reports it)
                 local = local <= a.halo_capacity ? local : 0;
-                const V4 q = pos[valid ? slot : first];
+                const V4 q = frame.place(pos[valid ? slot : first]);
                 for (int k = 0; k < centers; ++k)
                 {
                     const index_t center = first + k;
@@ -1028,11 +1090,11 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
                     if (hit && writing)
                     {
                         const index_t position = count + bits_below(mask);
-                        if (position < a.stride)
-                            a.index[center * a.stride + entry_position(position)] =
-                                static_cast<std::uint16_t>(local);
+                        std::uint16_t* row_out =
+                            a.index + static_cast<std::int64_t>(center) * a.stride;
+                        if (position < stride) row_out[position] = static_cast<std::uint16_t>(local);
                     }
-                    if (lane == k) mine_count += static_cast<index_t>(__popcll(mask));
+                    mine_count += lane == k ? static_cast<index_t>(__popcll(mask)) : 0;
                 }
             }
             // the centers' counts, the overflow flag, and the sentinel padding of each row
@@ -1052,8 +1114,8 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
                     std::uint16_t* row_out =
                         a.index + static_cast<std::int64_t>(first + k) * a.stride;
                     const index_t padded = (count + 31) & ~31;
-                    for (index_t p = count + lane; p < padded && p < a.stride; p += kWave)
-                        row_out[entry_position(p)] = 0;
+                    for (index_t p = count + lane; p < padded && p < stride; p += kWave)
+                        row_out[p] = 0;
                 }
         }
     }
@@ -1240,70 +1302,6 @@ __device__ __forceinline__ index_t brick_row_slot(index_t r, index_t runs,
     for (int step = kMaxBrickRuns / 2; step > 0; step >>= 1)
         if (run + step < runs && run_rows[run + step] <= r) run += step;
     return run_first[run] + (r - run_rows[run]);
-}
-
-// The periodic frame of a brick. Along each axis every position p is replaced by its image nearest
-// the brick, written so that a plain difference of two images is bit for bit the reference's
-// minimum-image displacement (space.py:11-19) of every pair within `reach` of the brick's atoms:
-// a brick in the lower half keeps p, or takes p - L for an atom more than L / 2 above its center
-// (then p > L / 2, so p - L is exact, and c - (p - L) is the reference's c - (o - L)); a brick
-// whose halo passes the upper face moves its whole frame down by L: p - L for every atom but
-// those more than L / 2 below its center, which keep p (p - L is exact for all atoms within
-// reach, which lie in the upper half, and (c - L) - p is the reference's (c - L) - o). The frame
-// exists only when every atom within reach of the brick lies within L / 2 of its center (and, for
-// the moved frame, in the upper half); otherwise `exact` is false and the walk takes the
-// reference's displacement per pair on unmoved positions.
-template <typename T>
-struct BrickFrame
-{
-    T center[3], length[3];
-    bool moved[3];
-    bool exact;
-
-    __device__ __forceinline__ T image(T p, int axis) const
-    {
-        const T offset = p - center[axis];
-        const T half = T(0.5) * length[axis];
-        if (moved[axis]) return offset < -half ? p : p - length[axis];
-        return offset > half ? p - length[axis] : p;
-    }
-
-    template <typename V4>
-    __device__ __forceinline__ V4 place(V4 p) const
-    {
-        if (!exact) return p;
-        p.x = image(p.x, 0);
-        p.y = image(p.y, 1);
-        p.z = image(p.z, 2);
-        return p;
-    }
-};
-
-template <typename T>
-__device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const BrickBox& box,
-                                                     T reach)
-{
-    BrickFrame<T> f;
-    f.exact = true;
-    const T widths[3] = {a.wx, a.wy, a.wz};
-    for (int axis = 0; axis < 3; ++axis)
-    {
-        const T w = widths[axis];
-        const T lo = static_cast<T>(box.lo[axis]) * w;
-        const T hi = static_cast<T>(box.hi[axis]) * w;
-        const T length = a.cell[axis];
-        f.center[axis] = T(0.5) * (lo + hi);
-        f.length[axis] = length;
-        // the brick's atoms may have left its sub-bins by up to half the list's skin since the
-        // build; a sub-bin of margin (at least half the list's radius) covers that and the
-        // rounding of the sub-bin bounds
-        const T extent = T(0.5) * (hi - lo) + reach + w;
-        f.moved[axis] = f.center[axis] + extent > length;
-        const bool inside = extent < T(0.5) * length;
-        const bool upper = !f.moved[axis] || f.center[axis] - extent > T(0.5) * length;
-        f.exact = f.exact && inside && upper;
-    }
-    return f;
 }
 
 // A brick staged by its block: the halo in LDS, placed in the brick's frame, and the brick's
@@ -1803,11 +1801,10 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     md_halo_kernel<T><<<gated_grid(a.bricks, kBlock / kWave), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_halo");
     const T r = static_cast<T>(radius);
-    const T r2 = r * r;
     // culling keeps every sub-bin a pair within R could sit in, with room for rounding
     const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
     md_fill_kernel<T><<<gated_grid(a.bins, kBlock / kWave), kBlock, 0, stream>>>(
-        a, r2, cull * cull, forced);
+        a, r, cull * cull, forced);
     NDN_LAUNCHED("md_fill");
     return NDN_OK;
 }

@@ -27,7 +27,7 @@ _CHUNK_STEPS = 25
 # Skin of the pruned list the force kernel reads (nm): the pairs of the full list within
 # cutoff + INNER_SKIN, refreshed whenever an atom has moved half of it. Only the force kernel's
 # work depends on it (the full list, its skin and its rebuild rule are the reference's).
-INNER_SKIN = 0.05
+INNER_SKIN = 0.08
 FIELDS = ("potential_energy", "kinetic_energy", "total_energy", "temperature", "pressure")
 
 
