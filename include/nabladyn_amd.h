@@ -2,8 +2,9 @@
  * nabladyn_amd — C ABI of the MI355X (gfx950) neighbor-list-and-pair-force library.
  *
  * Every entry point takes plain device pointers, element counts and a hipStream_t, launches
- * its kernels on that stream, never synchronises and never allocates (callers own every
- * buffer; sizes come from the *_bytes queries). Argument errors return NDN_ERR_ARGUMENT (or
+ * its kernels on that stream and never allocates (callers own every buffer; sizes come from the
+ * *_bytes queries and sizing calls); none synchronises but ndn_md_batch_prepare, which copies a
+ * host-built table. Argument errors return NDN_ERR_ARGUMENT (or
  * NDN_ERR_HIP for a launch failure) and leave a message for ndn_last_error(); errors in the
  * data (a bad cell, a non-finite coordinate, too many pairs) never fail a call: they set
  * latched bits in the list's device status word, as the reference does.
@@ -288,6 +289,51 @@ int ndn_md_refresh(int dtype, const ndn_md_system* system, const ndn_potential* 
  * reference counterpart (the reference's list holds atom indices already). */
 int ndn_md_list_entries(const ndn_md_list* md_list, int64_t num_atoms, int pruned, int32_t* out,
                         ndn_stream_t stream);
+
+/* ---- batches of MD systems ------------------------------------------------------------------
+ * Several systems stepped together: each kernel of ndn_md_step is launched once for the whole
+ * batch, each member getting the blocks its own step would launch, so a member computes the
+ * same bits as ndn_md_step gives it alone. No reference counterpart: the reference steps one
+ * system per compiled step (md/_simulation.py:316-364); a batch is that step for every member. */
+#define NDN_MD_BATCH_KERNELS 14
+
+/* One member: the same structures ndn_md_step takes. They are read at preparation only. */
+typedef struct ndn_md_member {
+    const ndn_md_system* system;
+    const ndn_md_settings* settings;
+    const ndn_potential* potential;
+    const ndn_list_state* list;
+    const ndn_md_list* md_list;
+} ndn_md_member;
+
+typedef struct ndn_md_batch {
+    void* table;          /* device memory of table_bytes bytes, or NULL to size it */
+    int64_t table_bytes;  /* set by a sizing call */
+    int32_t dtype;        /* NDN_F32 or NDN_F64: every member's */
+    /* set by ndn_md_batch_prepare */
+    int32_t count;
+    int32_t potential;    /* every member binds the same kind: LJ or a table, of one type or several */
+    int32_t group;        /* every member's lanes per row */
+    int32_t virial;       /* every member's ndn_md_settings.virial */
+    int32_t potential_lds;  /* LDS bytes a force block stages for the potential (one-type tables) */
+    int64_t halo_capacity;  /* the largest member's: the LDS of the brick kernels */
+    int64_t members_offset;
+    int64_t potentials_offset;
+    int64_t blocks_offset;
+    int64_t offsets[NDN_MD_BATCH_KERNELS];  /* each kernel's block map in the table */
+    int64_t grids[NDN_MD_BATCH_KERNELS];    /* each kernel's blocks over all members */
+} ndn_md_batch;
+
+/* Checks every member as ndn_md_step does (and that all share the dtype, the potential kind, the
+ * group and the virial setting; 1 <= count <= 256), lays out the batch's table and, when
+ * batch->table is set, writes it (synchronizing the stream). With batch->table NULL it sets
+ * table_bytes only. Prepare again after any member's list is resized. */
+int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int32_t count,
+                         ndn_stream_t stream);
+
+/* One step of every member (ndn_md_step's kernels, one launch each for the batch), with no sync
+ * or allocation. */
+int ndn_md_batch_step(const ndn_md_batch* batch, ndn_stream_t stream);
 
 /* Applies the pending closing velocity factor, leaving the velocities as the reference's frame
  * holds them after a step. */

@@ -79,10 +79,14 @@ def lib() -> ctypes.CDLL:
 
 def call(name: str, *args) -> None:
     """Call ``ndn_<name>``; raise ``NativeError`` with the library's message on failure."""
-    handle = lib()
-    code = getattr(handle, "ndn_" + name)(*args)
+    check(name, getattr(lib(), "ndn_" + name)(*args))
+
+
+def check(name: str, code: int) -> None:
+    """Raise ``NativeError`` with the library's message when ``ndn_<name>`` returned ``code``
+    other than 0 (for callers that hold the function itself)."""
     if code != 0:
-        message = handle.ndn_last_error().decode()
+        message = lib().ndn_last_error().decode()
         raise NativeError(f"ndn_{name}: {message}")
 
 

@@ -35,6 +35,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
+#include <vector>
 
 #include "common.hpp"
 #include "neighbors.hpp"
@@ -211,6 +213,116 @@ struct TypedTable
     }
 };
 
+// A table with one type: each block first writes every segment's cubic, (u1, a, b, c) of
+// Table::eval, into its LDS (after the halo and the brick's bookkeeping), so a pair reads one
+// aligned 16-byte (fp64: 32-byte) entry instead of gathering four knots; r and 1/r come from one
+// reciprocal square root instead of a square root and a division.
+template <typename T>
+struct OneTypeTable
+{
+    template <typename E>
+    static constexpr int waves = sizeof(E) == 4 ? 8 : 1;
+
+    using V4 = typename Vec4<T>::type;
+
+    const T* __restrict__ padded;  // [K + 2] (Table::padded of row 0)
+    const T* __restrict__ start;
+    const T* __restrict__ inverse_spacing;
+    int knots;
+
+    // Segments of the table: K - 1, the last starting at knot K - 2.
+    __host__ __device__ int segments() const { return knots - 1; }
+
+    __host__ __device__ std::size_t lds_bytes() const
+    {
+        return static_cast<std::size_t>(segments()) * sizeof(V4);
+    }
+
+    __device__ __forceinline__ void stage(unsigned char* lds) const
+    {
+        V4* out = reinterpret_cast<V4*>(lds);
+        for (int s = threadIdx.x; s < segments(); s += blockDim.x)
+        {
+            const T u0 = padded[s], u1 = padded[s + 1], u2 = padded[s + 2], u3 = padded[s + 3];
+            V4 k;
+            k.x = u1;
+            k.y = u2 - u0;
+            k.z = T(2) * u0 - T(5) * u1 + T(4) * u2 - u3;
+            k.w = T(3) * (u1 - u2) + u3 - u0;
+            out[s] = k;
+        }
+    }
+
+    struct Bound
+    {
+        const V4* segments;
+        T start, h_inverse, last;
+
+        __device__ __forceinline__ void eval(T, T, T r2, T& u, T& fs) const
+        {
+            const T inverse_r = rsqrt(r2);
+            const T r = r2 * inverse_r;
+            const T x = (r - start) * h_inverse;
+            T segment = floor(x);
+            segment = segment < T(0) ? T(0) : (segment > last ? last : segment);
+            const T t = x - segment;
+            // in range whatever r2 was (a NaN fails both clamps above)
+            const unsigned index = static_cast<unsigned>(static_cast<int>(segment));
+            const V4 k = segments[index <= static_cast<unsigned>(last) ? index : 0u];
+            const T s = t > T(0) ? t : T(0);
+            const T cubic = k.x + T(0.5) * s * (k.y + s * (k.z + s * k.w));
+            const bool below = t < T(0);
+            u = cubic + (below ? T(0.5) * k.y * t : T(0));
+            const T du_dt =
+                below ? T(0.5) * k.y : T(0.5) * (k.y + s * (T(2) * k.z + T(3) * s * k.w));
+            fs = -du_dt * h_inverse * inverse_r;
+        }
+    };
+
+    __device__ __forceinline__ Bound bind(const unsigned char* lds) const
+    {
+        return Bound{reinterpret_cast<const V4*>(lds), start[0], inverse_spacing[0],
+                     static_cast<T>(knots - 2)};
+    }
+};
+
+// LDS a potential's evaluator stages per block (after the halo and the brick's bookkeeping), its
+// staging, and its bound evaluator: nothing for the potentials read from registers or memory.
+template <class Pot>
+__host__ __device__ std::size_t potential_lds(const Pot&)
+{
+    return 0;
+}
+
+template <typename T>
+__host__ __device__ std::size_t potential_lds(const OneTypeTable<T>& pot)
+{
+    return pot.lds_bytes();
+}
+
+template <class Pot>
+__device__ __forceinline__ void stage_potential(const Pot&, unsigned char*)
+{
+}
+
+template <typename T>
+__device__ __forceinline__ void stage_potential(const OneTypeTable<T>& pot, unsigned char* lds)
+{
+    pot.stage(lds);
+}
+
+template <class Pot>
+__device__ __forceinline__ auto bind_potential(const Pot& pot, const unsigned char*)
+{
+    return pot.bind();
+}
+
+template <typename T>
+__device__ __forceinline__ auto bind_potential(const OneTypeTable<T>& pot, const unsigned char* lds)
+{
+    return pot.bind(lds);
+}
+
 template <typename T>
 struct StepArgs
 {
@@ -307,11 +419,25 @@ __device__ __forceinline__ bool refreshing(const StepArgs<T>& a)
            (a.rebuild[0] != 0 || prune_state(a.header)->refresh != 0);
 }
 
+// The block a kernel body runs as and the number of blocks of its system: the hardware's own in
+// a launch over one system, the system's share in a launch over a batch of systems (the bodies
+// read no blockIdx or gridDim, so a system computes the same bits alone and in a batch).
+struct Launch
+{
+    index_t block;
+    index_t blocks;
+};
+
+__device__ __forceinline__ Launch own_launch()
+{
+    return Launch{static_cast<index_t>(blockIdx.x), static_cast<index_t>(gridDim.x)};
+}
+
 // ---- prologue -------------------------------------------------------------------------------
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) prologue_kernel(StepArgs<T> a, const ndn_potential pot,
-                                                          void* coefficients)
+__device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_potential& pot,
+                                              void* coefficients)
 {
     if (threadIdx.x == 0)
     {
@@ -380,15 +506,22 @@ __global__ void __launch_bounds__(kBlock) prologue_kernel(StepArgs<T> a, const n
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kBlock) prologue_kernel(StepArgs<T> a, const ndn_potential pot,
+                                                          void* coefficients)
+{
+    prologue_body(a, pot, coefficients);
+}
+
 // ---- first half -----------------------------------------------------------------------------
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T timestep,
-                                                            T half_timestep)
+__device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Launch launch,
+                                                T timestep, T half_timestep)
 {
 #pragma clang fp contract(off)
     __shared__ T block_max[kBlock / kWave];
-    const std::int64_t atom = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::int64_t atom = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
     const bool check = a.status[0] == 0;
     bool finite = true;
     T moved = 0, moved_inner = 0;
@@ -495,8 +628,15 @@ __global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T tim
     {
         T m = block_max[0];
         for (int w = 1; w < kBlock / kWave; ++w) m = block_max[w] > m ? block_max[w] : m;
-        if (m > T(0)) atomicMax(moved_slot(a.header, blockIdx.x), order_bits(m));
+        if (m > T(0)) atomicMax(moved_slot(a.header, launch.block), order_bits(m));
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) first_half_kernel(StepArgs<T> a, T timestep,
+                                                            T half_timestep)
+{
+    first_half_body(a, own_launch(), timestep, half_timestep);
 }
 
 // ---- rebuild --------------------------------------------------------------------------------
@@ -529,11 +669,12 @@ __device__ __forceinline__ std::int64_t row_first_bin(index_t y, index_t z, inde
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_assign_kernel(StepArgs<T> a, int forced)
+__device__ __forceinline__ void md_assign_body(const StepArgs<T>& a, const Launch launch,
+                                               int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    for (std::int64_t slot = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
     {
         const T* p = a.positions + 3 * slot;
         const index_t bin = static_cast<index_t>(
@@ -545,16 +686,29 @@ __global__ void __launch_bounds__(kBlock) md_assign_kernel(StepArgs<T> a, int fo
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_scatter_kernel(StepArgs<T> a, int forced)
+__global__ void __launch_bounds__(kBlock) md_assign_kernel(StepArgs<T> a, int forced)
+{
+    md_assign_body(a, own_launch(), forced);
+}
+
+template <typename T>
+__device__ __forceinline__ void md_scatter_body(const StepArgs<T>& a, const Launch launch,
+                                                int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    for (std::int64_t slot = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
     {
         const index_t bin = a.bin_of[slot];
         a.unordered[a.bin_offsets[bin] + atomicSub(a.bin_counts + bin, 1) - 1] =
             static_cast<index_t>(slot);
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_scatter_kernel(StepArgs<T> a, int forced)
+{
+    md_scatter_body(a, own_launch(), forced);
 }
 
 // One atom of the frame, as the permutation moves it.
@@ -573,11 +727,12 @@ struct Packed
 // whatever the atomics did, and gathers its whole frame entry into scratch at its new slot;
 // md_unpack_kernel puts the frame home.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_order_kernel(StepArgs<T> a, int forced)
+__device__ __forceinline__ void md_order_body(const StepArgs<T>& a, const Launch launch,
+                                              int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         i < a.num_atoms; i += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    for (std::int64_t i = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
+         i < a.num_atoms; i += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
     {
         const std::int64_t old = a.unordered[i];
         const index_t bin = a.bin_of[old];
@@ -605,12 +760,19 @@ __global__ void __launch_bounds__(kBlock) md_order_kernel(StepArgs<T> a, int for
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int forced)
+__global__ void __launch_bounds__(kBlock) md_order_kernel(StepArgs<T> a, int forced)
+{
+    md_order_body(a, own_launch(), forced);
+}
+
+template <typename T>
+__device__ __forceinline__ void md_unpack_body(const StepArgs<T>& a, const Launch launch,
+                                               int forced)
 {
     using V4 = typename Vec4<T>::type;
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    for (std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         slot < a.num_atoms; slot += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+    for (std::int64_t slot = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
+         slot < a.num_atoms; slot += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
     {
         const Packed<T> p = reinterpret_cast<const Packed<T>*>(a.scratch)[slot];
         for (int k = 0; k < 3; ++k)
@@ -638,6 +800,12 @@ __global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int fo
             if (a.rebuild_count) ++a.rebuild_count[0];
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int forced)
+{
+    md_unpack_body(a, own_launch(), forced);
 }
 
 // Distance from coordinate c to the slab [lo, lo + w): 0 inside.
@@ -788,14 +956,15 @@ __device__ __forceinline__ Halo brick_halo(const BrickBox& box, const index_t di
 // count goes to the header (byte 656) for sizing; a count above the capacity sets
 // NDN_STATUS_HALO_OVERFLOW.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_halo_kernel(StepArgs<T> a, int forced)
+__device__ __forceinline__ void md_halo_body(const StepArgs<T>& a, const Launch launch,
+                                             int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
     const int lane = lane_id();
     const index_t dims[3] = {a.nx, a.ny, a.nz};
-    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (kBlock / kWave);
+    const std::int64_t waves = static_cast<std::int64_t>(launch.blocks) * (kBlock / kWave);
     for (std::int64_t brick =
-             static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+             static_cast<std::int64_t>(launch.block) * (kBlock / kWave) + threadIdx.x / kWave;
          brick < a.bricks; brick += waves)
     {
         const BrickBox box = brick_box(static_cast<index_t>(brick), dims, a.brick);
@@ -852,6 +1021,12 @@ __global__ void __launch_bounds__(kBlock) md_halo_kernel(StepArgs<T> a, int forc
                 atomicOr(a.status, kStatusHaloOverflow);
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_halo_kernel(StepArgs<T> a, int forced)
+{
+    md_halo_body(a, own_launch(), forced);
 }
 
 // The periodic frame of a brick. Along each axis every position p is replaced by its image nearest
@@ -928,8 +1103,8 @@ __device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const
 // Rows are then padded with the sentinel 0 up to their count rounded up to 32, so the force kernel
 // reads whole blocks unmasked. With stride 0 it only counts (the first frame, to size the rows).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius,
-                                                         T cull_squared, int forced)
+__device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch launch,
+                                             T radius, T cull_squared, int forced)
 {
 #pragma clang fp contract(off)
     using V4 = typename Vec4<T>::type;
@@ -947,9 +1122,9 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
     const bool writing = a.stride > 0;
     const index_t stride = static_cast<index_t>(a.stride);
     // one wave per sub-bin, the waves of the grid striding over the sub-bins
-    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (kBlock / kWave);
+    const std::int64_t waves = static_cast<std::int64_t>(launch.blocks) * (kBlock / kWave);
     for (std::int64_t bin =
-             static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+             static_cast<std::int64_t>(launch.block) * (kBlock / kWave) + threadIdx.x / kWave;
          bin < a.bins; bin += waves)
     {
         // the sub-bin is the wave's: its bounds, and every center's row address below, scalar
@@ -1121,6 +1296,13 @@ __global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius,
+                                                         T cull_squared, int forced)
+{
+    md_fill_body(a, own_launch(), radius, cull_squared, forced);
+}
+
 // ---- forces ---------------------------------------------------------------------------------
 
 inline unsigned xcd_grid(std::int64_t blocks)
@@ -1260,13 +1442,14 @@ __device__ __forceinline__ void walk_row(const Ev& ev, const V4& c, const V4* ha
     }
 }
 
-// The block of brick `b`: blocks are dealt to the 8 XCDs round-robin (blockIdx % 8), and the
+// The block of brick `b`: blocks are dealt to the 8 XCDs round-robin (block % 8: a batch's systems start
+// on multiples of 8), and the
 // blocks of one XCD take a contiguous eighth of the bricks, so bricks side by side, whose halos
 // overlap, share that XCD's L2. Blocks past the last brick idle.
-__device__ __forceinline__ index_t xcd_brick(index_t bricks)
+__device__ __forceinline__ index_t xcd_brick(index_t bricks, const Launch launch)
 {
     const index_t per_xcd = (bricks + 7) / 8;
-    return static_cast<index_t>(blockIdx.x % 8) * per_xcd + static_cast<index_t>(blockIdx.x / 8);
+    return (launch.block % 8) * per_xcd + launch.block / 8;
 }
 
 // The LDS of a brick's block, one dynamic allocation: the halo (positions and types; entry 0 the
@@ -1286,6 +1469,14 @@ template <typename T>
 __host__ __device__ __forceinline__ std::size_t halo_bytes(std::int64_t halo_capacity)
 {
     return static_cast<std::size_t>(halo_capacity + 1) * sizeof(typename Vec4<T>::type);
+}
+
+// Dynamic LDS of a brick's block: the halo (with the sentinel) and the block's bookkeeping (a
+// potential's staging follows).
+template <typename T>
+__host__ __device__ __forceinline__ std::size_t brick_block_bytes(std::int64_t halo_capacity)
+{
+    return halo_bytes<T>(halo_capacity) + kBrickSharedBytes;
 }
 
 template <typename T>
@@ -1375,7 +1566,8 @@ constexpr int kPruneGroup = 16;
 // number kept. Since every pair kept is in the full row, a pruned row never outgrows the stride.
 // The frame of the refresh is kept for the check. Gated on the refresh flag unless `forced`.
 template <typename T>
-__global__ void __launch_bounds__(kBrickThreads<T>) md_prune_kernel(StepArgs<T> a, int forced)
+__device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch launch,
+                                              int forced)
 {
 #pragma clang fp contract(off)
     using V4 = typename Vec4<T>::type;
@@ -1383,9 +1575,9 @@ __global__ void __launch_bounds__(kBrickThreads<T>) md_prune_kernel(StepArgs<T> 
     extern __shared__ __align__(16) unsigned char dynamic_shared[];
     V4* halo = reinterpret_cast<V4*>(dynamic_shared);
     BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
-    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks));
+    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks), launch);
     if (brick >= a.bricks) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
+    if (launch.block == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
     // images exact out to the pruned radius
     const StagedBrick<T> staged =
         stage_brick<kBrickThreads<T>>(a, brick, halo, sh, a.inner_radius);
@@ -1456,6 +1648,12 @@ __global__ void __launch_bounds__(kBrickThreads<T>) md_prune_kernel(StepArgs<T> 
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kBrickThreads<T>) md_prune_kernel(StepArgs<T> a, int forced)
+{
+    md_prune_body(a, own_launch(), forced);
+}
+
 // Waves per SIMD the force kernel is compiled for: each potential's `waves` (what its fp32
 // kernels without the virial reach without spilling). The virial's six more sums and fp64 need
 // more registers: those are left to the compiler (a block of kBrickThreads keeps at least 2
@@ -1475,17 +1673,17 @@ constexpr int kForceWaves = kVirial ? 1 : Pot::template waves<T>;
 // (space.py:11-19) per pair. The kick steps skip E unless they also sum W (nothing in a step
 // reads E).
 template <typename T, class Pot, int G, bool kVirial, bool kKick>
-__global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial>))
-    force_kernel(StepArgs<T> a, Pot pot, T cutoff, T cutoff_squared, T half_timestep,
-                 double* __restrict__ partials)
+__device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch launch,
+                                           const Pot& pot, T cutoff, T cutoff_squared,
+                                           T half_timestep, double* __restrict__ partials)
 {
     using V4 = typename Vec4<T>::type;
     constexpr bool kEnergy = !kKick || kVirial;
     extern __shared__ __align__(16) unsigned char dynamic_shared[];
     V4* halo = reinterpret_cast<V4*>(dynamic_shared);
     BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
-    double* out = partials + 8LL * blockIdx.x;
-    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks));
+    double* out = partials + 8LL * launch.block;
+    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks), launch);
     if (brick >= a.bricks)
     {
         if (threadIdx.x < 8) out[threadIdx.x] = 0.0;
@@ -1496,13 +1694,16 @@ __global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial
     const bool pruned = a.inner_index != nullptr;
     const index_t* counts = pruned ? a.inner_counts : a.counts;
     const std::uint16_t* index = pruned ? a.inner_index : a.index;
+    // the potential's own LDS after the brick's (the halo staging's barrier covers it)
+    unsigned char* potential_shared = dynamic_shared + brick_block_bytes<T>(a.halo_capacity);
+    stage_potential(pot, potential_shared);
     // images exact out to the cutoff
     const StagedBrick<T> staged = stage_brick<kBrickThreads<T>>(a, brick, halo, sh, cutoff);
     const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const int sub = lane_id() % G;
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const bool fast = staged.frame.exact;
-    const auto ev = pot.bind();
+    const auto ev = bind_potential(pot, potential_shared);
     double energy_sum = 0.0, kinetic = 0.0;
     double w_sum[6] = {0, 0, 0, 0, 0, 0};
     constexpr index_t kRowsPerPass = kBrickThreads<T> / G;
@@ -1569,6 +1770,15 @@ __global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial
     }
 }
 
+template <typename T, class Pot, int G, bool kVirial, bool kKick>
+__global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial>))
+    force_kernel(StepArgs<T> a, Pot pot, T cutoff, T cutoff_squared, T half_timestep,
+                 double* __restrict__ partials)
+{
+    force_body<T, Pot, G, kVirial, kKick>(a, own_launch(), pot, cutoff, cutoff_squared,
+                                          half_timestep, partials);
+}
+
 // The rows of the list as slots, for readout: out[slot * stride + k] = the slot's k-th listed
 // neighbor (in the row's own order), from its brick's halo; the row's entries past its
 // neighbors, up to its count rounded up to 32, get -1. One thread per slot.
@@ -1614,10 +1824,9 @@ constexpr int kFinishThreads = 1024;
 // E, W and K from the partials (pairs counted twice, so E and W halve); then the chain's
 // closing half step, whose velocity factor waits in the thermostat for the next step.
 template <typename T>
-__global__ void __launch_bounds__(kFinishThreads) finish_step_kernel(StepArgs<T> a,
-                                                                     const double* __restrict__ partials,
-                                                                     int blocks,
-                                                                     int kinetic_from_partials)
+__device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
+                                                 const double* __restrict__ partials, int blocks,
+                                                 int kinetic_from_partials)
 {
     __shared__ double column[kFinishThreads];
     // thread t sums partial (t % 8) of blocks t / 8, t / 8 + 128, ...: coalesced 8-byte reads,
@@ -1663,6 +1872,15 @@ __global__ void __launch_bounds__(kFinishThreads) finish_step_kernel(StepArgs<T>
         t[kPending] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
                                       a.degrees, a.kT, a.timestep);
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFinishThreads) finish_step_kernel(StepArgs<T> a,
+                                                                     const double* __restrict__ partials,
+                                                                     int blocks,
+                                                                     int kinetic_from_partials)
+{
+    finish_step_body(a, partials, blocks, kinetic_from_partials);
 }
 
 // Applies the pending closing factor to the velocities (before they are read out).
@@ -1809,13 +2027,6 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     return NDN_OK;
 }
 
-// Dynamic LDS of a brick's block: the halo (with the sentinel) and the block's bookkeeping.
-template <typename T>
-std::size_t brick_block_bytes(std::int64_t halo_capacity)
-{
-    return halo_bytes<T>(halo_capacity) + kBrickSharedBytes;
-}
-
 
 // The LDS a brick block's halo may take: what one block may hold, and no more than half a CU's
 // LDS (two blocks, 32 waves of fp32, share a CU), less the block's bookkeeping.
@@ -1834,6 +2045,17 @@ std::size_t max_halo_bytes()
     return bytes - kBrickSharedBytes;
 }
 
+// The LDS one block may take (a staged potential may go past max_halo_bytes' half a CU).
+std::size_t max_block_bytes()
+{
+    int device = 0, per_block = 65536;
+    if (hipGetDevice(&device) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+        hipSuccess)
+        return 65536;
+    return static_cast<std::size_t>(std::max(per_block, 65536));
+}
+
 // Kernels taking more than 64 KiB of dynamic LDS say so before their launch (a refusal shows
 // as the launch's own error).
 template <typename Kernel>
@@ -1850,7 +2072,7 @@ template <typename T, class Pot, int G, bool kVirial, bool kKick>
 void launch_force_kernel(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
                          T half_timestep, double* partials, unsigned grid, hipStream_t stream)
 {
-    const std::size_t lds = brick_block_bytes<T>(a.halo_capacity);
+    const std::size_t lds = brick_block_bytes<T>(a.halo_capacity) + potential_lds(pot);
     allow_shared(force_kernel<T, Pot, G, kVirial, kKick>, lds);
     force_kernel<T, Pot, G, kVirial, kKick><<<grid, kBrickThreads<T>, lds, stream>>>(
         a, pot, cutoff, cutoff_squared, half_timestep, partials);
@@ -1909,6 +2131,24 @@ inline unsigned force_grid(std::int64_t bricks)
 }
 
 template <typename T>
+OneTypeTable<T> bind_one_type_table(const ndn_potential* pot, void* coefficients)
+{
+    T* padded = static_cast<T*>(coefficients);
+    return OneTypeTable<T>{padded, static_cast<const T*>(pot->start), padded + (pot->knots + 2),
+                           static_cast<int>(pot->knots)};
+}
+
+// Whether a one-type table's segments are staged in LDS: when they fit beside the halo.
+template <typename T>
+bool stages_table(const ndn_potential* pot, std::int64_t halo_capacity)
+{
+    if (pot->kind == NDN_LJ || pot->num_types != 1) return false;
+    const std::size_t segments = static_cast<std::size_t>(pot->knots - 1) *
+                                 sizeof(typename Vec4<T>::type);
+    return brick_block_bytes<T>(halo_capacity) + segments <= max_block_bytes();
+}
+
+template <typename T>
 int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coefficients, int group,
                   bool virial, bool kick, double* partials, std::int64_t partial_slots,
                   hipStream_t stream, int* grid_out)
@@ -1937,6 +2177,11 @@ int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coeffici
         const TypedLJ<T> lj{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
         launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
                         partials, grid, stream);
+    }
+    else if (stages_table<T>(pot, a.halo_capacity))
+    {
+        launch_force<T>(a, bind_one_type_table<T>(pot, coefficients), group, cutoff,
+                        cutoff_squared, half_timestep, virial, kick, partials, grid, stream);
     }
     else
     {
@@ -2050,6 +2295,472 @@ int check_system(const ndn_md_system* s, const ndn_md_list* m)
                 "the halo capacity must be below 65535");
     NDN_REQUIRE(s->partial_slots >= static_cast<std::int64_t>(force_grid(bricks)),
                 "too few partial slots: one per brick, rounded up to a multiple of 8");
+    return NDN_OK;
+}
+
+// ---- batches of systems ---------------------------------------------------------------------
+//
+// A batch steps several systems with one launch per kernel of the step. Each kernel's blocks are
+// the members' blocks end to end, each member with as many as its own step would launch; a
+// block finds its member and its block within the member in the kernel's block map (one word
+// per block: member << 24 | block) and runs the same body with that member's arguments. Every
+// member therefore computes the bits it computes alone. The members, their bound potentials and
+// the block maps live in one device table written once by the batch's preparation.
+
+enum BatchKernel
+{
+    kBatchPrologue,
+    kBatchFirstHalf,
+    kBatchAssign,
+    kBatchTileSums,
+    kBatchTilePrefix,
+    kBatchTileApply,
+    kBatchScatter,
+    kBatchOrder,
+    kBatchUnpack,
+    kBatchHalo,
+    kBatchFill,
+    kBatchPrune,
+    kBatchForce,
+    kBatchFinish,
+    kBatchKernels
+};
+static_assert(kBatchKernels == NDN_MD_BATCH_KERNELS, "the header's kernel count");
+
+constexpr int kBatchMemberBits = 24;
+constexpr std::int64_t kBatchMaxMembers = 256;
+constexpr std::int64_t kBatchMaxBlocks = 1LL << kBatchMemberBits;
+
+// One member of a batch: its step arguments and the scalars its kernels take.
+template <typename T>
+struct BatchMember
+{
+    StepArgs<T> a;
+    ndn_potential pot;  // the prologue's coefficient tables
+    void* coefficients;
+    double* partials;
+    T radius;        // the list radius of its builds
+    T cull_squared;  // the fill's sub-bin culling
+    T timestep;
+    T half_timestep;
+    T cutoff;
+    T cutoff_squared;
+    index_t force_blocks;
+};
+
+// The member and its block of this kernel's block.
+struct BatchBlock
+{
+    index_t member;
+    Launch launch;
+};
+
+__device__ __forceinline__ BatchBlock batch_block(const std::uint32_t* __restrict__ map,
+                                                  const index_t* __restrict__ blocks, int kernel)
+{
+    const std::uint32_t word = map[blockIdx.x];
+    const index_t member = static_cast<index_t>(word >> kBatchMemberBits);
+    const index_t block = static_cast<index_t>(word & (kBatchMaxBlocks - 1));
+    return BatchBlock{member, Launch{block, blocks[member * kBatchKernels + kernel]}};
+}
+
+// The arguments of a batched launch: the table's members and block counts, and this kernel's map.
+template <typename T>
+struct BatchTable
+{
+    const BatchMember<T>* __restrict__ members;
+    const index_t* __restrict__ blocks;  // [member][kernel]
+    const std::uint32_t* __restrict__ map;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_prologue_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchPrologue);
+    const BatchMember<T>& m = t.members[b.member];
+    const StepArgs<T> a = m.a;
+    const ndn_potential pot = m.pot;
+    prologue_body(a, pot, m.coefficients);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_first_half_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchFirstHalf);
+    const BatchMember<T>& m = t.members[b.member];
+    const StepArgs<T> a = m.a;
+    first_half_body(a, b.launch, m.timestep, m.half_timestep);
+}
+
+// The gated build kernels taking only the arguments and the block.
+#define NDN_BATCH_GATED(name, kernel)                                                           \
+    template <typename T>                                                                        \
+    __global__ void __launch_bounds__(kBlock) batch_##name##_kernel(BatchTable<T> t)             \
+    {                                                                                            \
+        const BatchBlock b = batch_block(t.map, t.blocks, kernel);                            \
+        const StepArgs<T> a = t.members[b.member].a;                                             \
+        name##_body(a, b.launch, 0);                                                             \
+    }
+NDN_BATCH_GATED(md_assign, kBatchAssign)
+NDN_BATCH_GATED(md_scatter, kBatchScatter)
+NDN_BATCH_GATED(md_order, kBatchOrder)
+NDN_BATCH_GATED(md_unpack, kBatchUnpack)
+NDN_BATCH_GATED(md_halo, kBatchHalo)
+#undef NDN_BATCH_GATED
+
+// The build's scan of the sub-bin counts (bins + 1 values into bin_offsets), as exclusive_scan
+// runs it for one system.
+template <typename T>
+__device__ __forceinline__ index_t* batch_tile_sums(const StepArgs<T>& a)
+{
+    return reinterpret_cast<index_t*>(a.scan_temp + kHeaderBytes);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_tile_sums_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchTileSums);
+    const StepArgs<T> a = t.members[b.member].a;
+    if (!rebuilding(a.rebuild, a.status)) return;
+    scan_tile_sums_body(a.bin_counts, a.bins + 1, batch_tile_sums(a), b.launch.block);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_tile_prefix_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchTilePrefix);
+    const StepArgs<T> a = t.members[b.member].a;
+    if (!rebuilding(a.rebuild, a.status)) return;
+    scan_tile_prefix_body(batch_tile_sums(a), (a.bins + 1 + kScanTile - 1) / kScanTile);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_tile_apply_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchTileApply);
+    const StepArgs<T> a = t.members[b.member].a;
+    if (!rebuilding(a.rebuild, a.status)) return;
+    scan_apply_body(a.bin_counts, a.bins + 1, batch_tile_sums(a), a.bin_offsets, b.launch.block);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) batch_fill_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchFill);
+    const BatchMember<T>& m = t.members[b.member];
+    const StepArgs<T> a = m.a;
+    md_fill_body(a, b.launch, m.radius, m.cull_squared, 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBrickThreads<T>) batch_prune_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchPrune);
+    const StepArgs<T> a = t.members[b.member].a;
+    md_prune_body(a, b.launch, 0);
+}
+
+template <typename T, class Pot, int G, bool kVirial>
+__global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial>))
+    batch_force_kernel(BatchTable<T> t, const Pot* __restrict__ pots)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchForce);
+    const BatchMember<T>& m = t.members[b.member];
+    const StepArgs<T> a = m.a;
+    const Pot pot = pots[b.member];
+    force_body<T, Pot, G, kVirial, true>(a, b.launch, pot, m.cutoff, m.cutoff_squared,
+                                         m.half_timestep, m.partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFinishThreads) batch_finish_kernel(BatchTable<T> t)
+{
+    const BatchBlock b = batch_block(t.map, t.blocks, kBatchFinish);
+    const BatchMember<T>& m = t.members[b.member];
+    const StepArgs<T> a = m.a;
+    finish_step_body(a, m.partials, m.force_blocks, 1);
+}
+
+// The potentials of a batch: every member binds the same kind.
+enum BatchPotential
+{
+    kBatchOneTypeLJ,
+    kBatchTypedLJ,
+    kBatchTable,
+    kBatchOneTypeTable  // staged in LDS, when it fits beside the largest halo
+};
+
+int batch_potential(const ndn_potential* pot)
+{
+    if (pot->kind == NDN_LJ) return pot->num_types == 1 ? kBatchOneTypeLJ : kBatchTypedLJ;
+    return pot->num_types == 1 ? kBatchOneTypeTable : kBatchTable;
+}
+
+template <typename T>
+OneTypeLJ<T> bind_one_type_lj(const ndn_potential*, void* coefficients)
+{
+    return OneTypeLJ<T>{static_cast<const LJRow<T>*>(coefficients)};
+}
+
+template <typename T>
+TypedLJ<T> bind_typed_lj(const ndn_potential* pot, void* coefficients)
+{
+    return TypedLJ<T>{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
+}
+
+template <typename T>
+TypedTable<T> bind_table(const ndn_potential* pot, void* coefficients)
+{
+    const int rows = pot->num_types * (pot->num_types + 1) / 2;
+    T* padded = static_cast<T*>(coefficients);
+    return TypedTable<T>{Table<T>{padded, static_cast<const T*>(pot->start),
+                                  padded + static_cast<std::int64_t>(rows) * (pot->knots + 2),
+                                  nullptr, pot->num_types, static_cast<int>(pot->knots)}};
+}
+
+// The blocks a member's own step launches in each kernel (md_step_impl's grids).
+template <typename T>
+void member_blocks(const StepArgs<T>& a, index_t out[kBatchKernels])
+{
+    const std::int64_t n = a.num_atoms;
+    const std::int64_t tiles = scan_tiles(a.bins + 1);
+    out[kBatchPrologue] = 1;
+    out[kBatchFirstHalf] = static_cast<index_t>(blocks_for(n));
+    out[kBatchAssign] = static_cast<index_t>(gated_grid(n));
+    out[kBatchTileSums] = static_cast<index_t>(tiles);
+    out[kBatchTilePrefix] = 1;
+    out[kBatchTileApply] = static_cast<index_t>(tiles);
+    out[kBatchScatter] = static_cast<index_t>(gated_grid(n));
+    out[kBatchOrder] = static_cast<index_t>(gated_grid(n));
+    out[kBatchUnpack] = static_cast<index_t>(gated_grid(n));
+    out[kBatchHalo] = static_cast<index_t>(gated_grid(a.bricks, kBlock / kWave));
+    out[kBatchFill] = static_cast<index_t>(gated_grid(a.bins, kBlock / kWave));
+    out[kBatchPrune] = a.inner_index ? static_cast<index_t>(force_grid(a.bricks)) : 0;
+    out[kBatchForce] = static_cast<index_t>(force_grid(a.bricks));
+    out[kBatchFinish] = 1;
+}
+
+std::int64_t align_table(std::int64_t bytes)
+{
+    return (bytes + 255) / 256 * 256;
+}
+
+// Lays out and writes the table (when `batch->table` is set), and fills the batch's launch
+// fields. With no table it only sizes it.
+template <typename T, class Pot, Pot (*Bind)(const ndn_potential*, void*)>
+int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, int count,
+                  hipStream_t stream)
+{
+    std::vector<BatchMember<T>> hosts(static_cast<std::size_t>(count));
+    std::vector<Pot> pots(static_cast<std::size_t>(count));
+    std::vector<index_t> blocks(static_cast<std::size_t>(count) * kBatchKernels);
+    std::int64_t grids[kBatchKernels] = {};
+    std::int64_t halo_capacity = 1;
+    for (int i = 0; i < count; ++i)
+    {
+        const ndn_md_member& mb = members[i];
+        BatchMember<T>& h = hosts[i];
+        h.a = step_args<T>(*mb.system, mb.settings, mb.list, *mb.md_list);
+        h.pot = *mb.potential;
+        h.coefficients = mb.system->coefficients;
+        h.partials = mb.system->partials;
+        h.radius = static_cast<T>(mb.list->radius);
+        h.cull_squared = static_cast<T>(mb.list->radius * (1.0 + 1e-4) + 1e-6);
+        h.cull_squared = h.cull_squared * h.cull_squared;
+        h.timestep = static_cast<T>(mb.settings->timestep);
+        h.half_timestep = static_cast<T>(0.5 * mb.settings->timestep);
+        h.cutoff = static_cast<T>(mb.potential->cutoff);
+        h.cutoff_squared = static_cast<T>(mb.potential->cutoff * mb.potential->cutoff);
+        pots[i] = Bind(mb.potential, mb.system->coefficients);
+        index_t* own = blocks.data() + static_cast<std::size_t>(i) * kBatchKernels;
+        member_blocks<T>(h.a, own);
+        h.force_blocks = own[kBatchForce];
+        for (int k = 0; k < kBatchKernels; ++k)
+        {
+            NDN_REQUIRE(own[k] < kBatchMaxBlocks, "a member launches too many blocks for a batch");
+            grids[k] += own[k];
+        }
+        halo_capacity = std::max<std::int64_t>(halo_capacity, h.a.halo_capacity);
+    }
+    std::int64_t offset = align_table(sizeof(BatchMember<T>) * count);
+    const std::int64_t pots_at = offset;
+    offset += align_table(sizeof(Pot) * count);
+    const std::int64_t blocks_at = offset;
+    offset += align_table(sizeof(index_t) * count * kBatchKernels);
+    for (int k = 0; k < kBatchKernels; ++k)
+    {
+        NDN_REQUIRE(grids[k] <= INT32_MAX, "a batched launch would exceed 2**31 blocks");
+        batch->offsets[k] = offset;
+        offset += align_table(4 * std::max<std::int64_t>(grids[k], 1));
+    }
+    const std::int64_t needed = offset;
+    batch->members_offset = 0;
+    batch->potentials_offset = pots_at;
+    batch->blocks_offset = blocks_at;
+    for (int k = 0; k < kBatchKernels; ++k) batch->grids[k] = grids[k];
+    batch->halo_capacity = halo_capacity;
+    if (batch->table == nullptr)
+    {
+        batch->table_bytes = needed;
+        return NDN_OK;
+    }
+    NDN_REQUIRE(batch->table_bytes >= needed, "the batch table is too small");
+    std::vector<std::uint8_t> image(static_cast<std::size_t>(needed), 0);
+    std::memcpy(image.data(), hosts.data(), sizeof(BatchMember<T>) * count);
+    std::memcpy(image.data() + pots_at, pots.data(), sizeof(Pot) * count);
+    std::memcpy(image.data() + blocks_at, blocks.data(), sizeof(index_t) * count * kBatchKernels);
+    for (int k = 0; k < kBatchKernels; ++k)
+    {
+        auto* map = reinterpret_cast<std::uint32_t*>(image.data() + batch->offsets[k]);
+        std::int64_t at = 0;
+        for (int i = 0; i < count; ++i)
+            for (index_t block = 0; block < blocks[static_cast<std::size_t>(i) * kBatchKernels + k];
+                 ++block)
+                map[at++] = (static_cast<std::uint32_t>(i) << kBatchMemberBits) |
+                            static_cast<std::uint32_t>(block);
+    }
+    hipError_t error = hipMemcpyAsync(batch->table, image.data(), static_cast<std::size_t>(needed),
+                                      hipMemcpyHostToDevice, stream);
+    // the host image dies with this call
+    if (error == hipSuccess) error = hipStreamSynchronize(stream);
+    if (error != hipSuccess)
+    {
+        set_error(std::string("md_batch_prepare: ") + hipGetErrorString(error));
+        return NDN_ERR_HIP;
+    }
+    return NDN_OK;
+}
+
+template <typename T>
+BatchTable<T> batch_table(const ndn_md_batch& b, int kernel)
+{
+    auto* base = static_cast<std::uint8_t*>(b.table);
+    return BatchTable<T>{reinterpret_cast<const BatchMember<T>*>(base + b.members_offset),
+                         reinterpret_cast<const index_t*>(base + b.blocks_offset),
+                         reinterpret_cast<const std::uint32_t*>(base + b.offsets[kernel])};
+}
+
+template <typename T, class Pot, int G, bool kVirial>
+void launch_batch_force(const ndn_md_batch& b, hipStream_t stream)
+{
+    const std::size_t lds =
+        brick_block_bytes<T>(b.halo_capacity) + static_cast<std::size_t>(b.potential_lds);
+    allow_shared(batch_force_kernel<T, Pot, G, kVirial>, lds);
+    const Pot* pots =
+        reinterpret_cast<const Pot*>(static_cast<std::uint8_t*>(b.table) + b.potentials_offset);
+    batch_force_kernel<T, Pot, G, kVirial>
+        <<<static_cast<unsigned>(b.grids[kBatchForce]), kBrickThreads<T>, lds, stream>>>(
+            batch_table<T>(b, kBatchForce), pots);
+}
+
+template <typename T, class Pot>
+void launch_batch_force_of(const ndn_md_batch& b, hipStream_t stream)
+{
+    const bool virial = b.virial != 0;
+    switch (b.group)
+    {
+        case 4:
+            virial ? launch_batch_force<T, Pot, 4, true>(b, stream)
+                   : launch_batch_force<T, Pot, 4, false>(b, stream);
+            break;
+        case 8:
+            virial ? launch_batch_force<T, Pot, 8, true>(b, stream)
+                   : launch_batch_force<T, Pot, 8, false>(b, stream);
+            break;
+        default:
+            virial ? launch_batch_force<T, Pot, 16, true>(b, stream)
+                   : launch_batch_force<T, Pot, 16, false>(b, stream);
+    }
+}
+
+// One step of every member, in md_step_impl's order.
+template <typename T>
+int batch_step_impl(const ndn_md_batch& b, hipStream_t stream)
+{
+    auto grid = [&](int kernel) { return static_cast<unsigned>(b.grids[kernel]); };
+    batch_prologue_kernel<T><<<grid(kBatchPrologue), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchPrologue));
+    NDN_LAUNCHED("md_batch_prologue");
+    batch_first_half_kernel<T><<<grid(kBatchFirstHalf), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchFirstHalf));
+    NDN_LAUNCHED("md_batch_first_half");
+    batch_md_assign_kernel<T><<<grid(kBatchAssign), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchAssign));
+    NDN_LAUNCHED("md_batch_assign");
+    batch_tile_sums_kernel<T><<<grid(kBatchTileSums), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchTileSums));
+    NDN_LAUNCHED("md_batch_tile_sums");
+    batch_tile_prefix_kernel<T><<<grid(kBatchTilePrefix), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchTilePrefix));
+    NDN_LAUNCHED("md_batch_tile_prefix");
+    batch_tile_apply_kernel<T><<<grid(kBatchTileApply), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchTileApply));
+    NDN_LAUNCHED("md_batch_tile_apply");
+    batch_md_scatter_kernel<T><<<grid(kBatchScatter), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchScatter));
+    NDN_LAUNCHED("md_batch_scatter");
+    batch_md_order_kernel<T><<<grid(kBatchOrder), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchOrder));
+    NDN_LAUNCHED("md_batch_order");
+    batch_md_unpack_kernel<T><<<grid(kBatchUnpack), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchUnpack));
+    NDN_LAUNCHED("md_batch_unpack");
+    batch_md_halo_kernel<T><<<grid(kBatchHalo), kBlock, 0, stream>>>(
+        batch_table<T>(b, kBatchHalo));
+    NDN_LAUNCHED("md_batch_halo");
+    batch_fill_kernel<T><<<grid(kBatchFill), kBlock, 0, stream>>>(batch_table<T>(b, kBatchFill));
+    NDN_LAUNCHED("md_batch_fill");
+    if (b.grids[kBatchPrune] > 0)
+    {
+        const std::size_t lds = brick_block_bytes<T>(b.halo_capacity);
+        allow_shared(batch_prune_kernel<T>, lds);
+        batch_prune_kernel<T><<<grid(kBatchPrune), kBrickThreads<T>, lds, stream>>>(
+            batch_table<T>(b, kBatchPrune));
+        NDN_LAUNCHED("md_batch_prune");
+    }
+    switch (b.potential)
+    {
+        case kBatchOneTypeLJ:
+            launch_batch_force_of<T, OneTypeLJ<T>>(b, stream);
+            break;
+        case kBatchTypedLJ:
+            launch_batch_force_of<T, TypedLJ<T>>(b, stream);
+            break;
+        case kBatchOneTypeTable:
+            launch_batch_force_of<T, OneTypeTable<T>>(b, stream);
+            break;
+        default:
+            launch_batch_force_of<T, TypedTable<T>>(b, stream);
+    }
+    NDN_LAUNCHED("md_batch_force");
+    batch_finish_kernel<T><<<grid(kBatchFinish), kFinishThreads, 0, stream>>>(
+        batch_table<T>(b, kBatchFinish));
+    NDN_LAUNCHED("md_batch_finish");
+    return NDN_OK;
+}
+
+// What ndn_md_step checks of one system.
+int check_step(int dtype, const ndn_md_system* system, const ndn_md_settings* settings,
+               const ndn_potential* pot, const ndn_list_state* list, const ndn_md_list* md_list)
+{
+    int checked = check_system(system, md_list);
+    if (checked != NDN_OK) return checked;
+    checked = check_potential(pot, system->num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    NDN_REQUIRE(settings && list, "settings and list must not be null");
+    NDN_REQUIRE(settings->timestep > 0, "the timestep must be positive");
+    NDN_REQUIRE(!settings->thermostat || system->chain_masses, "NVT needs chain masses");
+    NDN_REQUIRE(list->cutoff == pot->cutoff, "the list's cutoff must be the potential's");
+    NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
+    NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
+                "several atom types need the slot-ordered types");
+    NDN_REQUIRE(list->rebuild != nullptr, "the step needs the rebuild flag");
+    const std::size_t halo = dtype == NDN_F32 ? halo_bytes<float>(md_list->halo_capacity)
+                                              : halo_bytes<double>(md_list->halo_capacity);
+    NDN_REQUIRE(md_list->halo_capacity >= 1 && halo <= max_halo_bytes(),
+                "the halo capacity must be at least 1 and fit the LDS of a block");
+    NDN_REQUIRE(md_list->stride >= 64, "the list stride must be a positive multiple of 64");
     return NDN_OK;
 }
 
@@ -2202,6 +2913,96 @@ int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* po
     return launch_forces<double>(a, pot, system->coefficients, md_list->group, virial != 0,
                                  kick != 0, system->partials, system->partial_slots, stream,
                                  &grid);
+}
+
+int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int32_t count,
+                         ndn_stream_t stream)
+{
+    NDN_REQUIRE(batch != nullptr && members != nullptr, "batch and members must not be null");
+    NDN_REQUIRE(count >= 1 && count <= kBatchMaxMembers, "a batch holds 1 to 256 members");
+    const int dtype = batch->dtype;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    for (int i = 0; i < count; ++i)
+    {
+        const ndn_md_member& m = members[i];
+        NDN_REQUIRE(m.system && m.settings && m.potential && m.list && m.md_list,
+                    "a member's structure is null");
+        const int checked = check_step(dtype, m.system, m.settings, m.potential, m.list, m.md_list);
+        if (checked != NDN_OK) return checked;
+        NDN_REQUIRE(m.md_list->group == members[0].md_list->group,
+                    "the members must share the group");
+        NDN_REQUIRE((m.settings->virial != 0) == (members[0].settings->virial != 0),
+                    "the members must share the virial setting");
+        NDN_REQUIRE(batch_potential(m.potential) == batch_potential(members[0].potential),
+                    "the members must bind the same kind of potential");
+    }
+    batch->count = count;
+    batch->potential = batch_potential(members[0].potential);
+    batch->group = members[0].md_list->group;
+    batch->virial = members[0].settings->virial != 0;
+    batch->potential_lds = 0;
+    if (batch->potential == kBatchOneTypeTable)
+    {
+        // staged when every member's segments fit beside the largest halo, else gathered
+        std::int64_t halo = 1, knots = 2;
+        for (int i = 0; i < count; ++i)
+        {
+            halo = std::max<std::int64_t>(halo, members[i].md_list->halo_capacity);
+            knots = std::max<std::int64_t>(knots, members[i].potential->knots);
+        }
+        const std::size_t point = dtype == NDN_F32 ? sizeof(float4) : sizeof(double4);
+        const std::size_t staged = static_cast<std::size_t>(knots - 1) * point;
+        const std::size_t brick = dtype == NDN_F32 ? brick_block_bytes<float>(halo)
+                                                   : brick_block_bytes<double>(halo);
+        if (brick + staged <= max_block_bytes())
+            batch->potential_lds = static_cast<std::int32_t>(staged);
+        else
+            batch->potential = kBatchTable;
+    }
+    if (dtype == NDN_F32)
+    {
+        switch (batch->potential)
+        {
+            case kBatchOneTypeTable:
+                return prepare_batch<float, OneTypeTable<float>, bind_one_type_table<float>>(
+                    batch, members, count, stream);
+            case kBatchOneTypeLJ:
+                return prepare_batch<float, OneTypeLJ<float>, bind_one_type_lj<float>>(
+                    batch, members, count, stream);
+            case kBatchTypedLJ:
+                return prepare_batch<float, TypedLJ<float>, bind_typed_lj<float>>(
+                    batch, members, count, stream);
+            default:
+                return prepare_batch<float, TypedTable<float>, bind_table<float>>(
+                    batch, members, count, stream);
+        }
+    }
+    switch (batch->potential)
+    {
+        case kBatchOneTypeTable:
+            return prepare_batch<double, OneTypeTable<double>, bind_one_type_table<double>>(
+                batch, members, count, stream);
+        case kBatchOneTypeLJ:
+            return prepare_batch<double, OneTypeLJ<double>, bind_one_type_lj<double>>(
+                batch, members, count, stream);
+        case kBatchTypedLJ:
+            return prepare_batch<double, TypedLJ<double>, bind_typed_lj<double>>(
+                batch, members, count, stream);
+        default:
+            return prepare_batch<double, TypedTable<double>, bind_table<double>>(
+                batch, members, count, stream);
+    }
+}
+
+int ndn_md_batch_step(const ndn_md_batch* batch, ndn_stream_t stream)
+{
+    NDN_REQUIRE(batch != nullptr && batch->table != nullptr && batch->count >= 1,
+                "the batch is not prepared");
+    NDN_REQUIRE(batch->grids[kBatchForce] >= 8 && batch->grids[kBatchFinish] == batch->count,
+                "the batch is not prepared");
+    if (batch->dtype == NDN_F32) return batch_step_impl<float>(*batch, stream);
+    NDN_REQUIRE(batch->dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    return batch_step_impl<double>(*batch, stream);
 }
 
 int ndn_md_settle(int dtype, const ndn_md_system* system, ndn_stream_t stream)

@@ -38,13 +38,14 @@ inline std::int64_t scan_bytes(std::int64_t items)
     return kHeaderBytes + 4 * (scan_tiles(items) + 1);
 }
 
-static __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
-    const index_t* __restrict__ input, std::int64_t count, index_t* __restrict__ tile_sums,
-    const index_t* rebuild, const index_t* status)
+// The bodies take the block (a launch over a batch of systems passes each system's own).
+static __device__ __forceinline__ void scan_tile_sums_body(const index_t* __restrict__ input,
+                                                           std::int64_t count,
+                                                           index_t* __restrict__ tile_sums,
+                                                           index_t block)
 {
-    if (!rebuilding(rebuild, status)) return;
     __shared__ index_t shared[16];
-    const std::int64_t base = static_cast<std::int64_t>(blockIdx.x) * kScanTile;
+    const std::int64_t base = static_cast<std::int64_t>(block) * kScanTile;
     index_t local = 0;
 #pragma unroll
     for (int k = 0; k < kScanPerThread; ++k)
@@ -53,14 +54,20 @@ static __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
         if (i < count) local += input[i];
     }
     const index_t sum = block_sum(local, shared);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = sum;
+    if (threadIdx.x == 0) tile_sums[block] = sum;
 }
 
-static __global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
-    index_t* __restrict__ tile_sums, std::int64_t tiles, const index_t* rebuild,
-    const index_t* status)
+static __global__ void __launch_bounds__(kBlock) scan_tile_sums_kernel(
+    const index_t* __restrict__ input, std::int64_t count, index_t* __restrict__ tile_sums,
+    const index_t* rebuild, const index_t* status)
 {
     if (!rebuilding(rebuild, status)) return;
+    scan_tile_sums_body(input, count, tile_sums, static_cast<index_t>(blockIdx.x));
+}
+
+static __device__ __forceinline__ void scan_tile_prefix_body(index_t* __restrict__ tile_sums,
+                                                             std::int64_t tiles)
+{
     __shared__ index_t shared[16];
     index_t carry = 0;
     for (std::int64_t base = 0; base < tiles; base += kScanTile)
@@ -88,15 +95,22 @@ static __global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
     }
 }
 
-// out may alias in: each block reads its whole tile before writing it.
-static __global__ void __launch_bounds__(kBlock) scan_apply_kernel(
-    const index_t* input, std::int64_t count, const index_t* __restrict__ tile_prefix,
-    index_t* output, const index_t* rebuild, const index_t* status)
+static __global__ void __launch_bounds__(kBlock) scan_tile_prefix_kernel(
+    index_t* __restrict__ tile_sums, std::int64_t tiles, const index_t* rebuild,
+    const index_t* status)
 {
     if (!rebuilding(rebuild, status)) return;
+    scan_tile_prefix_body(tile_sums, tiles);
+}
+
+// out may alias in: each block reads its whole tile before writing it.
+static __device__ __forceinline__ void scan_apply_body(const index_t* input, std::int64_t count,
+                                                       const index_t* __restrict__ tile_prefix,
+                                                       index_t* output, index_t block)
+{
     __shared__ index_t shared[16];
     const std::int64_t base =
-        static_cast<std::int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanPerThread;
+        static_cast<std::int64_t>(block) * kScanTile + threadIdx.x * kScanPerThread;
     index_t v[kScanPerThread];
     index_t local = 0;
 #pragma unroll
@@ -106,13 +120,21 @@ static __global__ void __launch_bounds__(kBlock) scan_apply_kernel(
         local += v[k];
     }
     index_t total;
-    index_t running = tile_prefix[blockIdx.x] + block_exclusive_sum(local, shared, &total);
+    index_t running = tile_prefix[block] + block_exclusive_sum(local, shared, &total);
 #pragma unroll
     for (int k = 0; k < kScanPerThread; ++k)
     {
         if (base + k < count) output[base + k] = running;
         running += v[k];
     }
+}
+
+static __global__ void __launch_bounds__(kBlock) scan_apply_kernel(
+    const index_t* input, std::int64_t count, const index_t* __restrict__ tile_prefix,
+    index_t* output, const index_t* rebuild, const index_t* status)
+{
+    if (!rebuilding(rebuild, status)) return;
+    scan_apply_body(input, count, tile_prefix, output, static_cast<index_t>(blockIdx.x));
 }
 
 // Exclusive scan of `count` values; returns NDN_OK or NDN_ERR_HIP.

@@ -357,7 +357,9 @@ class Simulation:
     """Molecular dynamics of a potential on the GPU; build it with ``nve`` or ``nvt``.
 
     It owns its frame, a neighbor list at the model's cutoff (with a full list for the fused
-    force kernel) and a HIP graph of one step, which ``run`` replays.
+    force kernel) and HIP graphs of one step and of 25, which ``run`` replays. With ``graphs``
+    False it captures none and launches each step as it comes (for a simulation stepped in a
+    ``SimulationBatch``, which saves the captures).
     """
 
     def __init__(self, model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor,
@@ -365,7 +367,7 @@ class Simulation:
                  images: Tensor | None, step: Integral, temperature: Real | None = None,
                  thermostat_timescale: Real | None = None, thermostat: ChainState | None = None,
                  virial_every_step: bool = False, force_group: int | None = None,
-                 inner_skin: Real | None = INNER_SKIN):
+                 inner_skin: Real | None = INNER_SKIN, graphs: bool = True):
         if not isinstance(model, Potential):
             raise TypeError("model must be a Potential")
         num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
@@ -475,8 +477,9 @@ class Simulation:
         self.kinetic.copy_(0.5 * (self.slot_masses.double()[:, None]
                                   * self.velocities.double().square()).sum())
         self._failure: str | None = None
-        self._graph = self._capture()
-        self._chunk_graph = self._capture(_CHUNK_STEPS)
+        # without graphs (a simulation stepped in a batch) each step is launched as it comes
+        self._graph = self._capture() if graphs else None
+        self._chunk_graph = self._capture(_CHUNK_STEPS) if graphs else None
 
     def _first_build(self) -> None:
         """Sort and count (rows and bricks' halos), size them, then fill. Bricks shrink when the
@@ -572,6 +575,10 @@ class Simulation:
     def _replay(self, steps: int) -> None:
         """Queues `steps` steps on the current stream: whole chunks through the multi-step
         graph, the rest one step at a time."""
+        if self._graph is None:
+            for _ in range(steps):
+                self._launch_step()
+            return
         chunks, rest = divmod(steps, _CHUNK_STEPS)
         for _ in range(chunks):
             self._chunk_graph.replay()

@@ -8,16 +8,18 @@ Requests arrive over time (``Server.submit``). Each scheduling round the server:
 2. launches one batched evaluation (``batch.Batch``: one binning pass, one scan, one force
    launch) for every energy/force request admitted since the last round, on the evaluation
    stream;
-3. advances every resident MD request by a chunk of steps — its own captured step graph,
-   replayed on one of the MD streams — so evaluations and MD segments overlap on the device;
+3. advances every resident MD request by a chunk of steps: the resident simulations are
+   grouped by ``md.batch_key`` into ``md.SimulationBatch`` es (one launch per kernel of the step
+   for the whole group), each group on its own MD stream, so evaluations and MD segments overlap
+   on the device and many small systems cost the launches of one;
 4. waits for the round's work, completes what finished and frees its memory.
 
 A request that arrives while others are on the GPU joins at the next round, as memory frees up.
 Latency is completion minus arrival, on the server's clock.
 
-Every result is bitwise the same alone and in the mix: the batch reduces each system in a fixed
-order, independent of the other systems, under a fixed schedule; each MD request runs its own
-simulation with a fixed force schedule.
+Every result is bitwise the same alone and in the mix: the evaluation batch reduces each system
+in a fixed order, independent of the other systems, under a fixed schedule; in an MD batch each
+member gets the blocks its own step launches and a fixed force schedule.
 
 There is no reference counterpart; the baseline is ``serve_plain``, each request alone on the
 plain path (a ``NeighborList`` plus ``quantity.energy_force_virial``, or ``md.nvt`` and ``run``).
@@ -80,16 +82,18 @@ class Request:
 
 
 class _Running:
-    """A resident MD request: its simulation, the steps left and its stream."""
+    """A resident MD request: its simulation and the steps left."""
 
-    def __init__(self, request: Request, simulation, stream):
-        self.request, self.simulation, self.stream = request, simulation, stream
+    def __init__(self, request: Request, simulation):
+        self.request, self.simulation = request, simulation
         self.left = request.steps
 
 
 class Server:
-    """See the module docstring. ``hbm_budget`` is in bytes; ``md_chunk`` steps per MD request
-    per round; ``md_streams`` streams for the MD requests (all at the default priority)."""
+    """See the module docstring. ``hbm_budget`` is in bytes; ``md_chunk`` steps at most per MD
+    request per round; ``md_streams`` streams for the MD batches (all at the default priority);
+    ``md_group`` the force schedule of every MD request (fixed, so results do not depend on
+    timing)."""
 
     def __init__(self, device, *, hbm_budget: int = 16 << 30, md_chunk: int = 100,
                  md_streams: int = 8, efw_schedule=("wave", 2), md_group: int = 8):
@@ -111,7 +115,7 @@ class Server:
         self.peak_resident_bytes = 0
         self.rounds = 0
         self._next_id = 0
-        self._next_stream = 0
+        self._batches: dict[tuple, md.SimulationBatch] = {}
 
     # ---- requests --------------------------------------------------------------------------
 
@@ -161,14 +165,12 @@ class Server:
                     capacity_multiplier=1.25, images=None, step=0,
                     temperature=request.temperature,
                     thermostat_timescale=request.thermostat_timescale,
-                    force_group=self.md_group)
+                    force_group=self.md_group, graphs=False)
                 measured = torch.cuda.memory_allocated(self.device) - before
                 if measured > need:  # account the real footprint
                     self.resident_bytes += measured - need
                     request.footprint = measured
-                stream = self.md_streams[self._next_stream % len(self.md_streams)]
-                self._next_stream += 1
-                started.append(_Running(request, simulation, stream))
+                started.append(_Running(request, simulation))
             self.peak_resident_bytes = max(self.peak_resident_bytes, self.resident_bytes)
         return evaluations, started
 
@@ -191,12 +193,22 @@ class Server:
                 for k, request in enumerate(evaluations):
                     request.result = {"energy": energy[k].clone(), "forces": parts[k].clone(),
                                       "virial": virial[k].clone()}
-        for run in self.running:
-            run.stream.wait_stream(producer)
-            with torch.cuda.stream(run.stream):
-                steps = min(self.md_chunk, run.left)
-                run.simulation.advance(steps)
+        # each group's batch is kept while its members stay and remade when they change
+        batches = {}
+        for k, group in enumerate(self._md_groups()):
+            stream = self.md_streams[k % len(self.md_streams)]
+            stream.wait_stream(producer)
+            steps = min(self.md_chunk, min(run.left for run in group))
+            key = tuple(id(run.simulation) for run in group)
+            with torch.cuda.stream(stream):
+                stepper = self._batches.get(key)
+                if stepper is None:  # its preparation syncs the stream
+                    stepper = md.SimulationBatch([run.simulation for run in group])
+                batches[key] = stepper
+                stepper.advance(steps)
+            for run in group:
                 run.left -= steps
+        self._batches = batches
         torch.cuda.synchronize(self.device)
         done = []
         if batch is not None:
@@ -220,6 +232,16 @@ class Server:
             self.resident_bytes -= request.footprint
         self.completed.extend(done)
         return done
+
+    def _md_groups(self) -> list[list[_Running]]:
+        """The resident MD requests by batch key, in admission order, at most 256 per group."""
+        groups: dict[tuple, list[_Running]] = {}
+        for run in self.running:
+            groups.setdefault(md.batch_key(run.simulation), []).append(run)
+        out = []
+        for members in groups.values():
+            out.extend(members[k:k + 256] for k in range(0, len(members), 256))
+        return out
 
     # ---- the loop --------------------------------------------------------------------------
 

@@ -30,6 +30,24 @@ def test_scan_temp_bytes_and_argument_errors():
                      ctypes.byref(buffers), None)
 
 
+def test_md_batch_argument_errors():
+    from nabladyn_amd.md import _batch
+
+    batch = _batch._Batch()
+    batch.dtype = _native.F32
+    members = (_batch._Member * 1)()
+    with pytest.raises(_native.NativeError, match="1 to 256"):
+        _native.call("md_batch_prepare", ctypes.byref(batch), members, 0, None)
+    with pytest.raises(_native.NativeError, match="structure is null"):
+        _native.call("md_batch_prepare", ctypes.byref(batch), members, 1, None)
+    batch.dtype = 7
+    with pytest.raises(_native.NativeError, match="dtype"):
+        _native.call("md_batch_prepare", ctypes.byref(batch), members, 1, None)
+    with pytest.raises(_native.NativeError, match="not prepared"):
+        _native.call("md_batch_step", ctypes.byref(_batch._Batch()), None)
+    assert ctypes.sizeof(_batch._Batch) == 8 * 2 + 4 * 6 + 8 * 4 + 8 * 2 * _batch._KERNELS
+
+
 def test_op_schemas_match_the_reference():
     ops = _ops.ops()
     reference = {
