@@ -292,9 +292,11 @@ int ndn_md_list_entries(const ndn_md_list* md_list, int64_t num_atoms, int prune
 
 /* ---- batches of MD systems ------------------------------------------------------------------
  * Several systems stepped together: each kernel of ndn_md_step is launched once for the whole
- * batch, each member getting the blocks its own step would launch, so a member computes the
- * same bits as ndn_md_step gives it alone. No reference counterpart: the reference steps one
- * system per compiled step (md/_simulation.py:316-364); a batch is that step for every member. */
+ * batch (the force kernel once for each kind of potential the members bind: LJ or a table, of
+ * one type or several), each member getting the blocks its own step would launch, so a member
+ * computes the same bits as ndn_md_step gives it alone. No reference counterpart: the reference
+ * steps one system per compiled step (md/_simulation.py:316-364); a batch is that step for
+ * every member. */
 #define NDN_MD_BATCH_KERNELS 14
 
 /* One member: the same structures ndn_md_step takes. They are read at preparation only. */
@@ -312,27 +314,28 @@ typedef struct ndn_md_batch {
     int32_t dtype;        /* NDN_F32 or NDN_F64: every member's */
     /* set by ndn_md_batch_prepare */
     int32_t count;
-    int32_t potential;    /* every member binds the same kind: LJ or a table, of one type or several */
+    int32_t kinds;        /* bit k: some member binds potential kind k (a force launch) */
     int32_t group;        /* every member's lanes per row */
     int32_t virial;       /* every member's ndn_md_settings.virial */
-    int32_t potential_lds;  /* LDS bytes a force block stages for the potential (one-type tables) */
-    int64_t halo_capacity;  /* the largest member's: the LDS of the brick kernels */
+    int32_t reserved;
     int64_t members_offset;
     int64_t potentials_offset;
     int64_t blocks_offset;
-    int64_t offsets[NDN_MD_BATCH_KERNELS];  /* each kernel's block map in the table */
-    int64_t grids[NDN_MD_BATCH_KERNELS];    /* each kernel's blocks over all members */
+    int64_t offsets[NDN_MD_BATCH_KERNELS];  /* each launch's block map in the table */
+    int64_t grids[NDN_MD_BATCH_KERNELS];    /* each launch's blocks over all members (0: skipped) */
+    int64_t lds[NDN_MD_BATCH_KERNELS];      /* each launch's dynamic LDS: its members' largest
+                                               halo (and a one-type table's staged segments) */
 } ndn_md_batch;
 
-/* Checks every member as ndn_md_step does (and that all share the dtype, the potential kind, the
- * group and the virial setting; 1 <= count <= 256), lays out the batch's table and, when
- * batch->table is set, writes it (synchronizing the stream). With batch->table NULL it sets
- * table_bytes only. Prepare again after any member's list is resized. */
+/* Checks every member as ndn_md_step does (and that all share the dtype, the group and the
+ * virial setting; 1 <= count <= 256), lays out the batch's table and, when batch->table is set,
+ * writes it (synchronizing the stream). With batch->table NULL it sets table_bytes only. Prepare
+ * again after any member's list is resized. */
 int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int32_t count,
                          ndn_stream_t stream);
 
-/* One step of every member (ndn_md_step's kernels, one launch each for the batch), with no sync
- * or allocation. */
+/* One step of every member (ndn_md_step's kernels, one launch each for the batch and one force
+ * launch per kind of potential), with no sync or allocation. */
 int ndn_md_batch_step(const ndn_md_batch* batch, ndn_stream_t stream);
 
 /* Applies the pending closing velocity factor, leaving the velocities as the reference's frame

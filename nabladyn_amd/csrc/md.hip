@@ -705,12 +705,6 @@ __device__ __forceinline__ void md_scatter_body(const StepArgs<T>& a, const Laun
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) md_scatter_kernel(StepArgs<T> a, int forced)
-{
-    md_scatter_body(a, own_launch(), forced);
-}
-
 // One atom of the frame, as the permutation moves it.
 template <typename T>
 struct Packed
@@ -1023,10 +1017,100 @@ __device__ __forceinline__ void md_halo_body(const StepArgs<T>& a, const Launch 
     }
 }
 
+// The build's exclusive scan of the sub-bin counts into bin_offsets (bins + 1 values) by one
+// block. A pass takes 16 rows of 1024 consecutive values, thread t holding value t of each row
+// (every load and store coalesced): each row is scanned within its waves, one wave scans the
+// pass's 16 x 16 wave totals in order, and every value adds its row's and wave's prefix; the
+// running total carries from pass to pass. One launch where a tiled scan takes three: the steps
+// without a build, nearly all, pay one empty launch.
+constexpr int kListScanThreads = 1024;
+constexpr int kListScanRows = 16;
+constexpr int kListScanWaves = kListScanThreads / kWave;
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_halo_kernel(StepArgs<T> a, int forced)
+__device__ __forceinline__ void md_scan_body(const StepArgs<T>& a)
 {
-    md_halo_body(a, own_launch(), forced);
+    // [row][wave] totals, scanned in place; then the pass total
+    __shared__ index_t totals[kListScanRows * kListScanWaves + 1];
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const std::int64_t count = a.bins + 1;
+    constexpr std::int64_t kPass = static_cast<std::int64_t>(kListScanThreads) * kListScanRows;
+    index_t carry = 0;
+    for (std::int64_t base = 0; base < count; base += kPass)
+    {
+        index_t v[kListScanRows], inclusive[kListScanRows];
+#pragma unroll
+        for (int j = 0; j < kListScanRows; ++j)
+        {
+            const std::int64_t at = base + static_cast<std::int64_t>(j) * kListScanThreads + threadIdx.x;
+            v[j] = at < count ? a.bin_counts[at] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kListScanRows; ++j) inclusive[j] = wave_inclusive_sum(v[j]);
+        if (lane == kWave - 1)
+#pragma unroll
+            for (int j = 0; j < kListScanRows; ++j) totals[j * kListScanWaves + wave] = inclusive[j];
+        __syncthreads();
+        if (wave == 0)
+        {
+            // 256 totals in (row, wave) order, 4 per lane
+            constexpr int kPer = kListScanRows * kListScanWaves / kWave;
+            index_t mine[kPer];
+            index_t sum = 0;
+#pragma unroll
+            for (int e = 0; e < kPer; ++e)
+            {
+                mine[e] = totals[lane * kPer + e];
+                sum += mine[e];
+            }
+            index_t before = wave_inclusive_sum(sum) - sum;
+#pragma unroll
+            for (int e = 0; e < kPer; ++e)
+            {
+                totals[lane * kPer + e] = before;
+                before += mine[e];
+            }
+            if (lane == kWave - 1) totals[kListScanRows * kListScanWaves] = before;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kListScanRows; ++j)
+        {
+            const std::int64_t at = base + static_cast<std::int64_t>(j) * kListScanThreads + threadIdx.x;
+            if (at < count)
+                a.bin_offsets[at] = carry + totals[j * kListScanWaves + wave] + inclusive[j] - v[j];
+        }
+        carry += totals[kListScanRows * kListScanWaves];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kListScanThreads) md_scan_kernel(StepArgs<T> a, int forced)
+{
+    if (!forced && !rebuilding(a.rebuild, a.status)) return;
+    md_scan_body(a);
+}
+
+// The scatter and the halo listing, both after the scan, in one launch: the first
+// `scatter_blocks` blocks scatter, the rest list the halos.
+template <typename T>
+__device__ __forceinline__ void md_scatter_halo_body(const StepArgs<T>& a, const Launch launch,
+                                                     index_t scatter_blocks, int forced)
+{
+    if (launch.block < scatter_blocks)
+        md_scatter_body(a, Launch{launch.block, scatter_blocks}, forced);
+    else
+        md_halo_body(a, Launch{launch.block - scatter_blocks, launch.blocks - scatter_blocks},
+                     forced);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_scatter_halo_kernel(StepArgs<T> a,
+                                                                 index_t scatter_blocks, int forced)
+{
+    md_scatter_halo_body(a, own_launch(), scatter_blocks, forced);
 }
 
 // The periodic frame of a brick. Along each axis every position p is replaced by its image nearest
@@ -2006,18 +2090,17 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     const std::int64_t n = a.num_atoms;
     md_assign_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_assign");
-    const index_t* gate = forced ? nullptr : a.rebuild;
-    const int scanned = exclusive_scan(a.bin_counts, a.bin_offsets, a.bins + 1, a.scan_temp, gate,
-                                       a.status, stream);
-    if (scanned != NDN_OK) return scanned;
-    md_scatter_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
-    NDN_LAUNCHED("md_scatter");
+    md_scan_kernel<T><<<1, kListScanThreads, 0, stream>>>(a, forced);
+    NDN_LAUNCHED("md_scan");
+    const unsigned scatter = gated_grid(n);
+    const unsigned halo = gated_grid(a.bricks, kBlock / kWave);
+    md_scatter_halo_kernel<T><<<scatter + halo, kBlock, 0, stream>>>(
+        a, static_cast<index_t>(scatter), forced);
+    NDN_LAUNCHED("md_scatter_halo");
     md_order_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_order");
     md_unpack_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_unpack");
-    md_halo_kernel<T><<<gated_grid(a.bricks, kBlock / kWave), kBlock, 0, stream>>>(a, forced);
-    NDN_LAUNCHED("md_halo");
     const T r = static_cast<T>(radius);
     // culling keeps every sub-bin a pair within R could sit in, with room for rounding
     const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
@@ -2300,32 +2383,69 @@ int check_system(const ndn_md_system* s, const ndn_md_list* m)
 
 // ---- batches of systems ---------------------------------------------------------------------
 //
-// A batch steps several systems with one launch per kernel of the step. Each kernel's blocks are
+// A batch steps several systems with one launch per kernel of the step. Each launch's blocks are
 // the members' blocks end to end, each member with as many as its own step would launch; a
-// block finds its member and its block within the member in the kernel's block map (one word
+// block finds its member and its block within the member in the launch's block map (one word
 // per block: member << 24 | block) and runs the same body with that member's arguments. Every
-// member therefore computes the bits it computes alone. The members, their bound potentials and
-// the block maps live in one device table written once by the batch's preparation.
+// member therefore computes the bits it computes alone. Only the force kernel depends on the
+// potential: it is launched once for each kind of potential the members bind, over those
+// members. The members, their bound potentials, the members' block counts of every part of the
+// step and the launches' block maps live in one device table written once by the preparation.
 
-enum BatchKernel
+// The parts of a step, each a body with its own block count per member.
+enum BatchPart
 {
-    kBatchPrologue,
-    kBatchFirstHalf,
-    kBatchAssign,
-    kBatchTileSums,
-    kBatchTilePrefix,
-    kBatchTileApply,
-    kBatchScatter,
-    kBatchOrder,
-    kBatchUnpack,
-    kBatchHalo,
-    kBatchFill,
-    kBatchPrune,
-    kBatchForce,
-    kBatchFinish,
-    kBatchKernels
+    kPartPrologue,
+    kPartFirstHalf,
+    kPartAssign,
+    kPartScan,
+    kPartScatter,
+    kPartHalo,
+    kPartOrder,
+    kPartUnpack,
+    kPartFill,
+    kPartPrune,
+    kPartForce,
+    kPartFinish,
+    kBatchParts
 };
-static_assert(kBatchKernels == NDN_MD_BATCH_KERNELS, "the header's kernel count");
+
+// The kinds of potential a member binds.
+enum BatchPotential
+{
+    kBatchOneTypeLJ,
+    kBatchTypedLJ,
+    kBatchTable,
+    kBatchOneTypeTable,  // staged in LDS, when it fits beside the batch's largest halo
+    kBatchPotentials
+};
+
+// The launches of a step: one per part but the scatter and the halo listing, which share one,
+// and the forces, one launch per kind of potential (kLaunchForce + kind).
+enum BatchLaunch
+{
+    kLaunchPrologue,
+    kLaunchFirstHalf,
+    kLaunchAssign,
+    kLaunchScan,
+    kLaunchScatterHalo,
+    kLaunchOrder,
+    kLaunchUnpack,
+    kLaunchFill,
+    kLaunchPrune,
+    kLaunchForce,
+    kLaunchFinish = kLaunchForce + kBatchPotentials,
+    kBatchLaunches
+};
+static_assert(kBatchLaunches == NDN_MD_BATCH_KERNELS, "the header's launch count");
+
+// The parts a launch runs, in block order within each member (kBatchParts: none).
+constexpr int kLaunchParts[kBatchLaunches][2] = {
+    {kPartPrologue, kBatchParts}, {kPartFirstHalf, kBatchParts}, {kPartAssign, kBatchParts},
+    {kPartScan, kBatchParts},     {kPartScatter, kPartHalo},     {kPartOrder, kBatchParts},
+    {kPartUnpack, kBatchParts},   {kPartFill, kBatchParts},      {kPartPrune, kBatchParts},
+    {kPartForce, kBatchParts},    {kPartForce, kBatchParts},     {kPartForce, kBatchParts},
+    {kPartForce, kBatchParts},    {kPartFinish, kBatchParts}};
 
 constexpr int kBatchMemberBits = 24;
 constexpr std::int64_t kBatchMaxMembers = 256;
@@ -2346,9 +2466,45 @@ struct BatchMember
     T cutoff;
     T cutoff_squared;
     index_t force_blocks;
+    index_t kind;  // BatchPotential
 };
 
-// The member and its block of this kernel's block.
+// A member's bound potential, in the field of its kind.
+template <typename T>
+struct MemberPotential
+{
+    OneTypeLJ<T> one_type_lj;
+    TypedLJ<T> typed_lj;
+    TypedTable<T> table;
+    OneTypeTable<T> one_type_table;
+};
+
+template <typename T>
+__host__ __device__ const OneTypeLJ<T>& pick(const MemberPotential<T>& p, const OneTypeLJ<T>*)
+{
+    return p.one_type_lj;
+}
+
+template <typename T>
+__host__ __device__ const TypedLJ<T>& pick(const MemberPotential<T>& p, const TypedLJ<T>*)
+{
+    return p.typed_lj;
+}
+
+template <typename T>
+__host__ __device__ const TypedTable<T>& pick(const MemberPotential<T>& p, const TypedTable<T>*)
+{
+    return p.table;
+}
+
+template <typename T>
+__host__ __device__ const OneTypeTable<T>& pick(const MemberPotential<T>& p,
+                                                const OneTypeTable<T>*)
+{
+    return p.one_type_table;
+}
+
+// The member and its block of this launch's block, with the member's blocks of `part`.
 struct BatchBlock
 {
     index_t member;
@@ -2356,27 +2512,27 @@ struct BatchBlock
 };
 
 __device__ __forceinline__ BatchBlock batch_block(const std::uint32_t* __restrict__ map,
-                                                  const index_t* __restrict__ blocks, int kernel)
+                                                  const index_t* __restrict__ blocks, int part)
 {
     const std::uint32_t word = map[blockIdx.x];
     const index_t member = static_cast<index_t>(word >> kBatchMemberBits);
     const index_t block = static_cast<index_t>(word & (kBatchMaxBlocks - 1));
-    return BatchBlock{member, Launch{block, blocks[member * kBatchKernels + kernel]}};
+    return BatchBlock{member, Launch{block, blocks[member * kBatchParts + part]}};
 }
 
-// The arguments of a batched launch: the table's members and block counts, and this kernel's map.
+// The arguments of a batched launch: the table's members and block counts, and this launch's map.
 template <typename T>
 struct BatchTable
 {
     const BatchMember<T>* __restrict__ members;
-    const index_t* __restrict__ blocks;  // [member][kernel]
+    const index_t* __restrict__ blocks;  // [member][part]
     const std::uint32_t* __restrict__ map;
 };
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_prologue_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchPrologue);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartPrologue);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
     const ndn_potential pot = m.pot;
@@ -2386,67 +2542,49 @@ __global__ void __launch_bounds__(kBlock) batch_prologue_kernel(BatchTable<T> t)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_first_half_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchFirstHalf);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartFirstHalf);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
     first_half_body(a, b.launch, m.timestep, m.half_timestep);
 }
 
 // The gated build kernels taking only the arguments and the block.
-#define NDN_BATCH_GATED(name, kernel)                                                           \
+#define NDN_BATCH_GATED(name, part)                                                             \
     template <typename T>                                                                        \
     __global__ void __launch_bounds__(kBlock) batch_##name##_kernel(BatchTable<T> t)             \
     {                                                                                            \
-        const BatchBlock b = batch_block(t.map, t.blocks, kernel);                            \
+        const BatchBlock b = batch_block(t.map, t.blocks, part);                                 \
         const StepArgs<T> a = t.members[b.member].a;                                             \
         name##_body(a, b.launch, 0);                                                             \
     }
-NDN_BATCH_GATED(md_assign, kBatchAssign)
-NDN_BATCH_GATED(md_scatter, kBatchScatter)
-NDN_BATCH_GATED(md_order, kBatchOrder)
-NDN_BATCH_GATED(md_unpack, kBatchUnpack)
-NDN_BATCH_GATED(md_halo, kBatchHalo)
+NDN_BATCH_GATED(md_assign, kPartAssign)
+NDN_BATCH_GATED(md_order, kPartOrder)
+NDN_BATCH_GATED(md_unpack, kPartUnpack)
 #undef NDN_BATCH_GATED
 
-// The build's scan of the sub-bin counts (bins + 1 values into bin_offsets), as exclusive_scan
-// runs it for one system.
 template <typename T>
-__device__ __forceinline__ index_t* batch_tile_sums(const StepArgs<T>& a)
+__global__ void __launch_bounds__(kListScanThreads) batch_scan_kernel(BatchTable<T> t)
 {
-    return reinterpret_cast<index_t*>(a.scan_temp + kHeaderBytes);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartScan);
+    const StepArgs<T> a = t.members[b.member].a;
+    if (!rebuilding(a.rebuild, a.status)) return;
+    md_scan_body(a);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) batch_tile_sums_kernel(BatchTable<T> t)
+__global__ void __launch_bounds__(kBlock) batch_scatter_halo_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchTileSums);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartScatter);
     const StepArgs<T> a = t.members[b.member].a;
-    if (!rebuilding(a.rebuild, a.status)) return;
-    scan_tile_sums_body(a.bin_counts, a.bins + 1, batch_tile_sums(a), b.launch.block);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) batch_tile_prefix_kernel(BatchTable<T> t)
-{
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchTilePrefix);
-    const StepArgs<T> a = t.members[b.member].a;
-    if (!rebuilding(a.rebuild, a.status)) return;
-    scan_tile_prefix_body(batch_tile_sums(a), (a.bins + 1 + kScanTile - 1) / kScanTile);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) batch_tile_apply_kernel(BatchTable<T> t)
-{
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchTileApply);
-    const StepArgs<T> a = t.members[b.member].a;
-    if (!rebuilding(a.rebuild, a.status)) return;
-    scan_apply_body(a.bin_counts, a.bins + 1, batch_tile_sums(a), a.bin_offsets, b.launch.block);
+    const index_t scatter = b.launch.blocks;
+    const index_t halo = t.blocks[b.member * kBatchParts + kPartHalo];
+    md_scatter_halo_body(a, Launch{b.launch.block, scatter + halo}, scatter, 0);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_fill_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchFill);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartFill);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
     md_fill_body(a, b.launch, m.radius, m.cull_squared, 0);
@@ -2455,19 +2593,20 @@ __global__ void __launch_bounds__(kBlock) batch_fill_kernel(BatchTable<T> t)
 template <typename T>
 __global__ void __launch_bounds__(kBrickThreads<T>) batch_prune_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchPrune);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartPrune);
     const StepArgs<T> a = t.members[b.member].a;
     md_prune_body(a, b.launch, 0);
 }
 
+// The forces of the members binding `Pot` (the launch's map holds only theirs).
 template <typename T, class Pot, int G, bool kVirial>
 __global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial>))
-    batch_force_kernel(BatchTable<T> t, const Pot* __restrict__ pots)
+    batch_force_kernel(BatchTable<T> t, const MemberPotential<T>* __restrict__ pots)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchForce);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartForce);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
-    const Pot pot = pots[b.member];
+    const Pot pot = pick(pots[b.member], static_cast<const Pot*>(nullptr));
     force_body<T, Pot, G, kVirial, true>(a, b.launch, pot, m.cutoff, m.cutoff_squared,
                                          m.half_timestep, m.partials);
 }
@@ -2475,37 +2614,16 @@ __global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial
 template <typename T>
 __global__ void __launch_bounds__(kFinishThreads) batch_finish_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kBatchFinish);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartFinish);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
     finish_step_body(a, m.partials, m.force_blocks, 1);
 }
 
-// The potentials of a batch: every member binds the same kind.
-enum BatchPotential
-{
-    kBatchOneTypeLJ,
-    kBatchTypedLJ,
-    kBatchTable,
-    kBatchOneTypeTable  // staged in LDS, when it fits beside the largest halo
-};
-
 int batch_potential(const ndn_potential* pot)
 {
     if (pot->kind == NDN_LJ) return pot->num_types == 1 ? kBatchOneTypeLJ : kBatchTypedLJ;
     return pot->num_types == 1 ? kBatchOneTypeTable : kBatchTable;
-}
-
-template <typename T>
-OneTypeLJ<T> bind_one_type_lj(const ndn_potential*, void* coefficients)
-{
-    return OneTypeLJ<T>{static_cast<const LJRow<T>*>(coefficients)};
-}
-
-template <typename T>
-TypedLJ<T> bind_typed_lj(const ndn_potential* pot, void* coefficients)
-{
-    return TypedLJ<T>{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
 }
 
 template <typename T>
@@ -2518,26 +2636,54 @@ TypedTable<T> bind_table(const ndn_potential* pot, void* coefficients)
                                   nullptr, pot->num_types, static_cast<int>(pot->knots)}};
 }
 
-// The blocks a member's own step launches in each kernel (md_step_impl's grids).
 template <typename T>
-void member_blocks(const StepArgs<T>& a, index_t out[kBatchKernels])
+MemberPotential<T> bind_member(const ndn_potential* pot, void* coefficients, int kind)
+{
+    MemberPotential<T> p{};
+    switch (kind)
+    {
+        case kBatchOneTypeLJ:
+            p.one_type_lj = OneTypeLJ<T>{static_cast<const LJRow<T>*>(coefficients)};
+            break;
+        case kBatchTypedLJ:
+            p.typed_lj = TypedLJ<T>{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
+            break;
+        case kBatchOneTypeTable:
+            p.one_type_table = bind_one_type_table<T>(pot, coefficients);
+            break;
+        default:
+            p.table = bind_table<T>(pot, coefficients);
+    }
+    return p;
+}
+
+// The blocks a member's own step launches for each part (md_step_impl's grids).
+template <typename T>
+void member_blocks(const StepArgs<T>& a, index_t out[kBatchParts])
 {
     const std::int64_t n = a.num_atoms;
-    const std::int64_t tiles = scan_tiles(a.bins + 1);
-    out[kBatchPrologue] = 1;
-    out[kBatchFirstHalf] = static_cast<index_t>(blocks_for(n));
-    out[kBatchAssign] = static_cast<index_t>(gated_grid(n));
-    out[kBatchTileSums] = static_cast<index_t>(tiles);
-    out[kBatchTilePrefix] = 1;
-    out[kBatchTileApply] = static_cast<index_t>(tiles);
-    out[kBatchScatter] = static_cast<index_t>(gated_grid(n));
-    out[kBatchOrder] = static_cast<index_t>(gated_grid(n));
-    out[kBatchUnpack] = static_cast<index_t>(gated_grid(n));
-    out[kBatchHalo] = static_cast<index_t>(gated_grid(a.bricks, kBlock / kWave));
-    out[kBatchFill] = static_cast<index_t>(gated_grid(a.bins, kBlock / kWave));
-    out[kBatchPrune] = a.inner_index ? static_cast<index_t>(force_grid(a.bricks)) : 0;
-    out[kBatchForce] = static_cast<index_t>(force_grid(a.bricks));
-    out[kBatchFinish] = 1;
+    out[kPartPrologue] = 1;
+    out[kPartFirstHalf] = static_cast<index_t>(blocks_for(n));
+    out[kPartAssign] = static_cast<index_t>(gated_grid(n));
+    out[kPartScan] = 1;
+    out[kPartScatter] = static_cast<index_t>(gated_grid(n));
+    out[kPartHalo] = static_cast<index_t>(gated_grid(a.bricks, kBlock / kWave));
+    out[kPartOrder] = static_cast<index_t>(gated_grid(n));
+    out[kPartUnpack] = static_cast<index_t>(gated_grid(n));
+    out[kPartFill] = static_cast<index_t>(gated_grid(a.bins, kBlock / kWave));
+    out[kPartPrune] = a.inner_index ? static_cast<index_t>(force_grid(a.bricks)) : 0;
+    out[kPartForce] = static_cast<index_t>(force_grid(a.bricks));
+    out[kPartFinish] = 1;
+}
+
+// A member's blocks in a launch (a force launch holds only the members of its kind).
+std::int64_t launch_blocks(const index_t* own, int kind, int launch)
+{
+    if (launch >= kLaunchForce && launch < kLaunchFinish && launch - kLaunchForce != kind) return 0;
+    std::int64_t blocks = 0;
+    for (const int part : kLaunchParts[launch])
+        if (part != kBatchParts) blocks += own[part];
+    return blocks;
 }
 
 std::int64_t align_table(std::int64_t bytes)
@@ -2546,16 +2692,17 @@ std::int64_t align_table(std::int64_t bytes)
 }
 
 // Lays out and writes the table (when `batch->table` is set), and fills the batch's launch
-// fields. With no table it only sizes it.
-template <typename T, class Pot, Pot (*Bind)(const ndn_potential*, void*)>
-int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, int count,
-                  hipStream_t stream)
+// fields. With no table it only sizes it. `kinds` holds each member's BatchPotential.
+template <typename T>
+int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, const int* kinds, int count,
+                  std::int64_t staged_bytes, hipStream_t stream)
 {
     std::vector<BatchMember<T>> hosts(static_cast<std::size_t>(count));
-    std::vector<Pot> pots(static_cast<std::size_t>(count));
-    std::vector<index_t> blocks(static_cast<std::size_t>(count) * kBatchKernels);
-    std::int64_t grids[kBatchKernels] = {};
-    std::int64_t halo_capacity = 1;
+    std::vector<MemberPotential<T>> pots(static_cast<std::size_t>(count));
+    std::vector<index_t> blocks(static_cast<std::size_t>(count) * kBatchParts);
+    std::int64_t grids[kBatchLaunches] = {};
+    // the largest halo of each launch's members (the brick kernels' LDS)
+    std::int64_t halos[kBatchLaunches] = {};
     for (int i = 0; i < count; ++i)
     {
         const ndn_md_member& mb = members[i];
@@ -2571,23 +2718,25 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, int count,
         h.half_timestep = static_cast<T>(0.5 * mb.settings->timestep);
         h.cutoff = static_cast<T>(mb.potential->cutoff);
         h.cutoff_squared = static_cast<T>(mb.potential->cutoff * mb.potential->cutoff);
-        pots[i] = Bind(mb.potential, mb.system->coefficients);
-        index_t* own = blocks.data() + static_cast<std::size_t>(i) * kBatchKernels;
+        h.kind = kinds[i];
+        pots[i] = bind_member<T>(mb.potential, mb.system->coefficients, kinds[i]);
+        index_t* own = blocks.data() + static_cast<std::size_t>(i) * kBatchParts;
         member_blocks<T>(h.a, own);
-        h.force_blocks = own[kBatchForce];
-        for (int k = 0; k < kBatchKernels; ++k)
+        h.force_blocks = own[kPartForce];
+        for (int k = 0; k < kBatchLaunches; ++k)
         {
-            NDN_REQUIRE(own[k] < kBatchMaxBlocks, "a member launches too many blocks for a batch");
-            grids[k] += own[k];
+            const std::int64_t launched = launch_blocks(own, kinds[i], k);
+            NDN_REQUIRE(launched < kBatchMaxBlocks, "a member launches too many blocks for a batch");
+            grids[k] += launched;
+            if (launched > 0) halos[k] = std::max<std::int64_t>(halos[k], h.a.halo_capacity);
         }
-        halo_capacity = std::max<std::int64_t>(halo_capacity, h.a.halo_capacity);
     }
     std::int64_t offset = align_table(sizeof(BatchMember<T>) * count);
     const std::int64_t pots_at = offset;
-    offset += align_table(sizeof(Pot) * count);
+    offset += align_table(sizeof(MemberPotential<T>) * count);
     const std::int64_t blocks_at = offset;
-    offset += align_table(sizeof(index_t) * count * kBatchKernels);
-    for (int k = 0; k < kBatchKernels; ++k)
+    offset += align_table(sizeof(index_t) * count * kBatchParts);
+    for (int k = 0; k < kBatchLaunches; ++k)
     {
         NDN_REQUIRE(grids[k] <= INT32_MAX, "a batched launch would exceed 2**31 blocks");
         batch->offsets[k] = offset;
@@ -2597,8 +2746,18 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, int count,
     batch->members_offset = 0;
     batch->potentials_offset = pots_at;
     batch->blocks_offset = blocks_at;
-    for (int k = 0; k < kBatchKernels; ++k) batch->grids[k] = grids[k];
-    batch->halo_capacity = halo_capacity;
+    for (int k = 0; k < kBatchLaunches; ++k)
+    {
+        batch->grids[k] = grids[k];
+        batch->lds[k] = 0;
+    }
+    batch->lds[kLaunchPrune] = static_cast<std::int64_t>(brick_block_bytes<T>(halos[kLaunchPrune]));
+    for (int kind = 0; kind < kBatchPotentials; ++kind)
+    {
+        const int launch = kLaunchForce + kind;
+        batch->lds[launch] = static_cast<std::int64_t>(brick_block_bytes<T>(halos[launch])) +
+                             (kind == kBatchOneTypeTable ? staged_bytes : 0);
+    }
     if (batch->table == nullptr)
     {
         batch->table_bytes = needed;
@@ -2607,17 +2766,25 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, int count,
     NDN_REQUIRE(batch->table_bytes >= needed, "the batch table is too small");
     std::vector<std::uint8_t> image(static_cast<std::size_t>(needed), 0);
     std::memcpy(image.data(), hosts.data(), sizeof(BatchMember<T>) * count);
-    std::memcpy(image.data() + pots_at, pots.data(), sizeof(Pot) * count);
-    std::memcpy(image.data() + blocks_at, blocks.data(), sizeof(index_t) * count * kBatchKernels);
-    for (int k = 0; k < kBatchKernels; ++k)
+    std::memcpy(image.data() + pots_at, pots.data(), sizeof(MemberPotential<T>) * count);
+    std::memcpy(image.data() + blocks_at, blocks.data(), sizeof(index_t) * count * kBatchParts);
+    for (int k = 0; k < kBatchLaunches; ++k)
     {
         auto* map = reinterpret_cast<std::uint32_t*>(image.data() + batch->offsets[k]);
         std::int64_t at = 0;
         for (int i = 0; i < count; ++i)
-            for (index_t block = 0; block < blocks[static_cast<std::size_t>(i) * kBatchKernels + k];
-                 ++block)
+        {
+            const std::int64_t launched =
+                launch_blocks(blocks.data() + static_cast<std::size_t>(i) * kBatchParts, kinds[i], k);
+            for (std::int64_t block = 0; block < launched; ++block)
                 map[at++] = (static_cast<std::uint32_t>(i) << kBatchMemberBits) |
                             static_cast<std::uint32_t>(block);
+        }
+        if (at != grids[k])
+        {
+            set_error("md_batch_prepare: a block map does not cover its launch");
+            return NDN_ERR_ARGUMENT;
+        }
     }
     hipError_t error = hipMemcpyAsync(batch->table, image.data(), static_cast<std::size_t>(needed),
                                       hipMemcpyHostToDevice, stream);
@@ -2632,44 +2799,44 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, int count,
 }
 
 template <typename T>
-BatchTable<T> batch_table(const ndn_md_batch& b, int kernel)
+BatchTable<T> batch_table(const ndn_md_batch& b, int launch)
 {
     auto* base = static_cast<std::uint8_t*>(b.table);
     return BatchTable<T>{reinterpret_cast<const BatchMember<T>*>(base + b.members_offset),
                          reinterpret_cast<const index_t*>(base + b.blocks_offset),
-                         reinterpret_cast<const std::uint32_t*>(base + b.offsets[kernel])};
+                         reinterpret_cast<const std::uint32_t*>(base + b.offsets[launch])};
 }
 
 template <typename T, class Pot, int G, bool kVirial>
-void launch_batch_force(const ndn_md_batch& b, hipStream_t stream)
+void launch_batch_force(const ndn_md_batch& b, int launch, hipStream_t stream)
 {
-    const std::size_t lds =
-        brick_block_bytes<T>(b.halo_capacity) + static_cast<std::size_t>(b.potential_lds);
+    const std::size_t lds = static_cast<std::size_t>(b.lds[launch]);
     allow_shared(batch_force_kernel<T, Pot, G, kVirial>, lds);
-    const Pot* pots =
-        reinterpret_cast<const Pot*>(static_cast<std::uint8_t*>(b.table) + b.potentials_offset);
+    const auto* pots = reinterpret_cast<const MemberPotential<T>*>(
+        static_cast<std::uint8_t*>(b.table) + b.potentials_offset);
     batch_force_kernel<T, Pot, G, kVirial>
-        <<<static_cast<unsigned>(b.grids[kBatchForce]), kBrickThreads<T>, lds, stream>>>(
-            batch_table<T>(b, kBatchForce), pots);
+        <<<static_cast<unsigned>(b.grids[launch]), kBrickThreads<T>, lds, stream>>>(
+            batch_table<T>(b, launch), pots);
 }
 
 template <typename T, class Pot>
-void launch_batch_force_of(const ndn_md_batch& b, hipStream_t stream)
+void launch_batch_force_of(const ndn_md_batch& b, int kind, hipStream_t stream)
 {
+    const int launch = kLaunchForce + kind;
     const bool virial = b.virial != 0;
     switch (b.group)
     {
         case 4:
-            virial ? launch_batch_force<T, Pot, 4, true>(b, stream)
-                   : launch_batch_force<T, Pot, 4, false>(b, stream);
+            virial ? launch_batch_force<T, Pot, 4, true>(b, launch, stream)
+                   : launch_batch_force<T, Pot, 4, false>(b, launch, stream);
             break;
         case 8:
-            virial ? launch_batch_force<T, Pot, 8, true>(b, stream)
-                   : launch_batch_force<T, Pot, 8, false>(b, stream);
+            virial ? launch_batch_force<T, Pot, 8, true>(b, launch, stream)
+                   : launch_batch_force<T, Pot, 8, false>(b, launch, stream);
             break;
         default:
-            virial ? launch_batch_force<T, Pot, 16, true>(b, stream)
-                   : launch_batch_force<T, Pot, 16, false>(b, stream);
+            virial ? launch_batch_force<T, Pot, 16, true>(b, launch, stream)
+                   : launch_batch_force<T, Pot, 16, false>(b, launch, stream);
     }
 }
 
@@ -2677,64 +2844,60 @@ void launch_batch_force_of(const ndn_md_batch& b, hipStream_t stream)
 template <typename T>
 int batch_step_impl(const ndn_md_batch& b, hipStream_t stream)
 {
-    auto grid = [&](int kernel) { return static_cast<unsigned>(b.grids[kernel]); };
-    batch_prologue_kernel<T><<<grid(kBatchPrologue), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchPrologue));
+    auto grid = [&](int launch) { return static_cast<unsigned>(b.grids[launch]); };
+    batch_prologue_kernel<T><<<grid(kLaunchPrologue), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchPrologue));
     NDN_LAUNCHED("md_batch_prologue");
-    batch_first_half_kernel<T><<<grid(kBatchFirstHalf), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchFirstHalf));
+    batch_first_half_kernel<T><<<grid(kLaunchFirstHalf), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchFirstHalf));
     NDN_LAUNCHED("md_batch_first_half");
-    batch_md_assign_kernel<T><<<grid(kBatchAssign), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchAssign));
+    batch_md_assign_kernel<T><<<grid(kLaunchAssign), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchAssign));
     NDN_LAUNCHED("md_batch_assign");
-    batch_tile_sums_kernel<T><<<grid(kBatchTileSums), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchTileSums));
-    NDN_LAUNCHED("md_batch_tile_sums");
-    batch_tile_prefix_kernel<T><<<grid(kBatchTilePrefix), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchTilePrefix));
-    NDN_LAUNCHED("md_batch_tile_prefix");
-    batch_tile_apply_kernel<T><<<grid(kBatchTileApply), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchTileApply));
-    NDN_LAUNCHED("md_batch_tile_apply");
-    batch_md_scatter_kernel<T><<<grid(kBatchScatter), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchScatter));
-    NDN_LAUNCHED("md_batch_scatter");
-    batch_md_order_kernel<T><<<grid(kBatchOrder), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchOrder));
+    batch_scan_kernel<T><<<grid(kLaunchScan), kListScanThreads, 0, stream>>>(
+        batch_table<T>(b, kLaunchScan));
+    NDN_LAUNCHED("md_batch_scan");
+    batch_scatter_halo_kernel<T><<<grid(kLaunchScatterHalo), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchScatterHalo));
+    NDN_LAUNCHED("md_batch_scatter_halo");
+    batch_md_order_kernel<T><<<grid(kLaunchOrder), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchOrder));
     NDN_LAUNCHED("md_batch_order");
-    batch_md_unpack_kernel<T><<<grid(kBatchUnpack), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchUnpack));
+    batch_md_unpack_kernel<T><<<grid(kLaunchUnpack), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchUnpack));
     NDN_LAUNCHED("md_batch_unpack");
-    batch_md_halo_kernel<T><<<grid(kBatchHalo), kBlock, 0, stream>>>(
-        batch_table<T>(b, kBatchHalo));
-    NDN_LAUNCHED("md_batch_halo");
-    batch_fill_kernel<T><<<grid(kBatchFill), kBlock, 0, stream>>>(batch_table<T>(b, kBatchFill));
+    batch_fill_kernel<T><<<grid(kLaunchFill), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchFill));
     NDN_LAUNCHED("md_batch_fill");
-    if (b.grids[kBatchPrune] > 0)
+    if (b.grids[kLaunchPrune] > 0)
     {
-        const std::size_t lds = brick_block_bytes<T>(b.halo_capacity);
+        const std::size_t lds = static_cast<std::size_t>(b.lds[kLaunchPrune]);
         allow_shared(batch_prune_kernel<T>, lds);
-        batch_prune_kernel<T><<<grid(kBatchPrune), kBrickThreads<T>, lds, stream>>>(
-            batch_table<T>(b, kBatchPrune));
+        batch_prune_kernel<T><<<grid(kLaunchPrune), kBrickThreads<T>, lds, stream>>>(
+            batch_table<T>(b, kLaunchPrune));
         NDN_LAUNCHED("md_batch_prune");
     }
-    switch (b.potential)
+    for (int kind = 0; kind < kBatchPotentials; ++kind)
     {
-        case kBatchOneTypeLJ:
-            launch_batch_force_of<T, OneTypeLJ<T>>(b, stream);
-            break;
-        case kBatchTypedLJ:
-            launch_batch_force_of<T, TypedLJ<T>>(b, stream);
-            break;
-        case kBatchOneTypeTable:
-            launch_batch_force_of<T, OneTypeTable<T>>(b, stream);
-            break;
-        default:
-            launch_batch_force_of<T, TypedTable<T>>(b, stream);
+        if (b.grids[kLaunchForce + kind] == 0) continue;
+        switch (kind)
+        {
+            case kBatchOneTypeLJ:
+                launch_batch_force_of<T, OneTypeLJ<T>>(b, kind, stream);
+                break;
+            case kBatchTypedLJ:
+                launch_batch_force_of<T, TypedLJ<T>>(b, kind, stream);
+                break;
+            case kBatchOneTypeTable:
+                launch_batch_force_of<T, OneTypeTable<T>>(b, kind, stream);
+                break;
+            default:
+                launch_batch_force_of<T, TypedTable<T>>(b, kind, stream);
+        }
+        NDN_LAUNCHED("md_batch_force");
     }
-    NDN_LAUNCHED("md_batch_force");
-    batch_finish_kernel<T><<<grid(kBatchFinish), kFinishThreads, 0, stream>>>(
-        batch_table<T>(b, kBatchFinish));
+    batch_finish_kernel<T><<<grid(kLaunchFinish), kFinishThreads, 0, stream>>>(
+        batch_table<T>(b, kLaunchFinish));
     NDN_LAUNCHED("md_batch_finish");
     return NDN_OK;
 }
@@ -2922,6 +3085,7 @@ int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int3
     NDN_REQUIRE(count >= 1 && count <= kBatchMaxMembers, "a batch holds 1 to 256 members");
     const int dtype = batch->dtype;
     NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    std::int64_t halo = 1, knots = 2;
     for (int i = 0; i < count; ++i)
     {
         const ndn_md_member& m = members[i];
@@ -2933,72 +3097,43 @@ int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int3
                     "the members must share the group");
         NDN_REQUIRE((m.settings->virial != 0) == (members[0].settings->virial != 0),
                     "the members must share the virial setting");
-        NDN_REQUIRE(batch_potential(m.potential) == batch_potential(members[0].potential),
-                    "the members must bind the same kind of potential");
+        halo = std::max<std::int64_t>(halo, m.md_list->halo_capacity);
+        if (batch_potential(m.potential) == kBatchOneTypeTable)
+            knots = std::max<std::int64_t>(knots, m.potential->knots);
+    }
+    // one-type tables are staged when the largest fits beside the batch's largest halo, else
+    // gathered like typed tables
+    const std::size_t point = dtype == NDN_F32 ? sizeof(float4) : sizeof(double4);
+    const std::size_t staged = static_cast<std::size_t>(knots - 1) * point;
+    const std::size_t brick =
+        dtype == NDN_F32 ? brick_block_bytes<float>(halo) : brick_block_bytes<double>(halo);
+    const bool stage = brick + staged <= max_block_bytes();
+    std::vector<int> kinds(static_cast<std::size_t>(count));
+    batch->kinds = 0;
+    for (int i = 0; i < count; ++i)
+    {
+        int kind = batch_potential(members[i].potential);
+        if (kind == kBatchOneTypeTable && !stage) kind = kBatchTable;
+        kinds[i] = kind;
+        batch->kinds |= 1 << kind;
     }
     batch->count = count;
-    batch->potential = batch_potential(members[0].potential);
     batch->group = members[0].md_list->group;
     batch->virial = members[0].settings->virial != 0;
-    batch->potential_lds = 0;
-    if (batch->potential == kBatchOneTypeTable)
-    {
-        // staged when every member's segments fit beside the largest halo, else gathered
-        std::int64_t halo = 1, knots = 2;
-        for (int i = 0; i < count; ++i)
-        {
-            halo = std::max<std::int64_t>(halo, members[i].md_list->halo_capacity);
-            knots = std::max<std::int64_t>(knots, members[i].potential->knots);
-        }
-        const std::size_t point = dtype == NDN_F32 ? sizeof(float4) : sizeof(double4);
-        const std::size_t staged = static_cast<std::size_t>(knots - 1) * point;
-        const std::size_t brick = dtype == NDN_F32 ? brick_block_bytes<float>(halo)
-                                                   : brick_block_bytes<double>(halo);
-        if (brick + staged <= max_block_bytes())
-            batch->potential_lds = static_cast<std::int32_t>(staged);
-        else
-            batch->potential = kBatchTable;
-    }
+    batch->reserved = 0;
+    const std::int64_t staged_bytes = static_cast<std::int64_t>(staged);
     if (dtype == NDN_F32)
-    {
-        switch (batch->potential)
-        {
-            case kBatchOneTypeTable:
-                return prepare_batch<float, OneTypeTable<float>, bind_one_type_table<float>>(
-                    batch, members, count, stream);
-            case kBatchOneTypeLJ:
-                return prepare_batch<float, OneTypeLJ<float>, bind_one_type_lj<float>>(
-                    batch, members, count, stream);
-            case kBatchTypedLJ:
-                return prepare_batch<float, TypedLJ<float>, bind_typed_lj<float>>(
-                    batch, members, count, stream);
-            default:
-                return prepare_batch<float, TypedTable<float>, bind_table<float>>(
-                    batch, members, count, stream);
-        }
-    }
-    switch (batch->potential)
-    {
-        case kBatchOneTypeTable:
-            return prepare_batch<double, OneTypeTable<double>, bind_one_type_table<double>>(
-                batch, members, count, stream);
-        case kBatchOneTypeLJ:
-            return prepare_batch<double, OneTypeLJ<double>, bind_one_type_lj<double>>(
-                batch, members, count, stream);
-        case kBatchTypedLJ:
-            return prepare_batch<double, TypedLJ<double>, bind_typed_lj<double>>(
-                batch, members, count, stream);
-        default:
-            return prepare_batch<double, TypedTable<double>, bind_table<double>>(
-                batch, members, count, stream);
-    }
+        return prepare_batch<float>(batch, members, kinds.data(), count, staged_bytes, stream);
+    return prepare_batch<double>(batch, members, kinds.data(), count, staged_bytes, stream);
 }
 
 int ndn_md_batch_step(const ndn_md_batch* batch, ndn_stream_t stream)
 {
     NDN_REQUIRE(batch != nullptr && batch->table != nullptr && batch->count >= 1,
                 "the batch is not prepared");
-    NDN_REQUIRE(batch->grids[kBatchForce] >= 8 && batch->grids[kBatchFinish] == batch->count,
+    std::int64_t force = 0;
+    for (int kind = 0; kind < kBatchPotentials; ++kind) force += batch->grids[kLaunchForce + kind];
+    NDN_REQUIRE(force >= 8 && batch->grids[kLaunchFinish] == batch->count,
                 "the batch is not prepared");
     if (batch->dtype == NDN_F32) return batch_step_impl<float>(*batch, stream);
     NDN_REQUIRE(batch->dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");

@@ -2,8 +2,10 @@
 
 A member computes the same bits in a batch as alone (``Simulation.run``): each kernel gives every
 member the blocks its own step launches, and each block runs the single-system code on its
-member's data. What a batch saves is launches: a step of ``k`` small systems costs the step's
-14 launches once instead of ``k`` times, so many small systems run at the rate of one large one.
+member's data. The force kernel, the only one that depends on the potential, is launched once for
+each kind the members bind (LJ or a table, of one type or several). What a batch saves is
+launches: a step of ``k`` small systems costs the step's 11 (or so) launches once instead of
+``k`` times, so many small systems run at the rate of one large one.
 
 There is no reference counterpart: the reference compiles and replays one system's step
 (``md/_simulation.py:316-364``); a batch is that step for every member at once.
@@ -39,21 +41,20 @@ class _Batch(_native.ctypes.Structure):
     _fields_ = [
         ("table", _native.ctypes.c_void_p), ("table_bytes", _native.ctypes.c_int64),
         ("dtype", _native.ctypes.c_int32), ("count", _native.ctypes.c_int32),
-        ("potential", _native.ctypes.c_int32), ("group", _native.ctypes.c_int32),
-        ("virial", _native.ctypes.c_int32), ("potential_lds", _native.ctypes.c_int32),
-        ("halo_capacity", _native.ctypes.c_int64), ("members_offset", _native.ctypes.c_int64),
+        ("kinds", _native.ctypes.c_int32), ("group", _native.ctypes.c_int32),
+        ("virial", _native.ctypes.c_int32), ("reserved", _native.ctypes.c_int32),
+        ("members_offset", _native.ctypes.c_int64),
         ("potentials_offset", _native.ctypes.c_int64), ("blocks_offset", _native.ctypes.c_int64),
         ("offsets", _native.ctypes.c_int64 * _KERNELS), ("grids", _native.ctypes.c_int64 * _KERNELS),
+        ("lds", _native.ctypes.c_int64 * _KERNELS),
     ]
 
 
 def batch_key(simulation: Simulation) -> tuple:
-    """Simulations with equal keys can share a batch: the same device and dtype, the same kind of
-    bound potential (LJ or a table, of one type or several), the same force schedule and virial
-    setting."""
-    pot = simulation._pot
-    kind = ("lj" if pot.kind == _native.LJ else "table") + ("1" if pot.num_types == 1 else "")
-    return (str(simulation._device), simulation._dtype, kind, simulation.group,
+    """Simulations with equal keys can share a batch: the same device and dtype, the same force
+    schedule and virial setting (the potentials may differ: each kind present gets its own force
+    launch)."""
+    return (str(simulation._device), simulation._dtype, simulation.group,
             bool(simulation._virial_every_step))
 
 
@@ -74,8 +75,8 @@ class SimulationBatch:
         keys = {batch_key(s) for s in simulations}
         if len(keys) != 1:
             raise ValueError(
-                "the simulations of a batch must share the device, dtype, potential kind, "
-                f"force group and virial setting; got {sorted(map(str, keys))}")
+                "the simulations of a batch must share the device, dtype, force group and "
+                f"virial setting; got {sorted(map(str, keys))}")
         for simulation in simulations:
             simulation._require()
         first = simulations[0]

@@ -9,9 +9,9 @@ Requests arrive over time (``Server.submit``). Each scheduling round the server:
    launch) for every energy/force request admitted since the last round, on the evaluation
    stream;
 3. advances every resident MD request by a chunk of steps: the resident simulations are
-   grouped by ``md.batch_key`` into ``md.SimulationBatch`` es (one launch per kernel of the step
-   for the whole group), each group on its own MD stream, so evaluations and MD segments overlap
-   on the device and many small systems cost the launches of one;
+   grouped by ``md.batch_key`` and kind of potential into ``md.SimulationBatch`` es (one launch
+   per kernel of the step for the whole group), each group on its own MD stream, so evaluations
+   and MD segments overlap on the device and many small systems cost the launches of one;
 4. waits for the round's work, completes what finished and frees its memory.
 
 A request that arrives while others are on the GPU joins at the next round, as memory frees up.
@@ -234,10 +234,15 @@ class Server:
         return done
 
     def _md_groups(self) -> list[list[_Running]]:
-        """The resident MD requests by batch key, in admission order, at most 256 per group."""
+        """The resident MD requests by batch key and kind of potential, in admission order, at
+        most 256 per group: one batch per kind, each on its own stream, so the kinds' steps
+        overlap on the device (one batch of mixed kinds runs its force launches one after the
+        other, with the LDS of the largest halo in its shared launches)."""
         groups: dict[tuple, list[_Running]] = {}
         for run in self.running:
-            groups.setdefault(md.batch_key(run.simulation), []).append(run)
+            pot = run.simulation._pot
+            key = md.batch_key(run.simulation) + (pot.kind, pot.num_types == 1)
+            groups.setdefault(key, []).append(run)
         out = []
         for members in groups.values():
             out.extend(members[k:k + 256] for k in range(0, len(members), 256))

@@ -1,10 +1,10 @@
 """Batched MD (``md.SimulationBatch``): every member of a batch ends bitwise where the same
 simulation run alone (``Simulation.run``, one captured graph per step) ends.
 
-Members: fp32 and fp64 LJ argon and 200-knot table systems of 125 to 4,096 atoms in boxes of
-2.5 to 5.9 nm at 8 to 21 nm^-3 (some boxes short against 2 x (cutoff + skin), one member
-without a pruned list, NVE and NVT mixed), stepped far enough that each rebuilds its list
-several times. The bar is bitwise: a member's kernels get the blocks its own step launches and
+Members: fp32 and fp64 LJ argon and 200-knot table systems (by kind and mixed in one batch)
+of 125 to 4,096 atoms in boxes of 2.5 to 5.9 nm at 8 to 21 nm^-3 (some boxes short against
+2 x (cutoff + skin), one member without a pruned list, NVE and NVT mixed), stepped far enough
+that each rebuilds its list several times. The bar is bitwise: a member's kernels get the blocks its own step launches and
 run the same code on its data.
 """
 
@@ -92,18 +92,31 @@ def test_batch_then_alone_continues(device):
         np.testing.assert_equal(a.sample()["temperature"], b.sample()["temperature"])
 
 
+def test_mixed_potentials_in_one_batch(device):
+    # LJ and one-type-table members share every launch but their force launches
+    makers = _members("lj", torch.float32, device)[:3] + _members("table", torch.float32, device)[2:]
+    alone = [make() for make in makers]
+    for s in alone:
+        s.run(90)
+    batched = [make(graphs=False) for make in makers]
+    batch = md.SimulationBatch(batched)
+    assert batch.grids[9] > 0 and batch.grids[12] > 0  # one-type LJ and one-type table forces
+    batch.advance(90)
+    batch.check()
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        assert _same(a, b), k
+
+
 def test_batch_refuses_mixed_and_empty(device):
     lj = _members("lj", torch.float32, device)[0](graphs=False)
-    table = _members("table", torch.float32, device)[0](graphs=False)
     wide = _members("lj", torch.float64, device)[0](graphs=False)
     with pytest.raises(ValueError):
         md.SimulationBatch([])
     with pytest.raises(ValueError):
-        md.SimulationBatch([lj, table])
-    with pytest.raises(ValueError):
         md.SimulationBatch([lj, wide])
     with pytest.raises(ValueError):
         md.SimulationBatch([lj, lj])
-    assert md.batch_key(lj) != md.batch_key(table)
+    table = _members("table", torch.float32, device)[0](graphs=False)
+    assert md.batch_key(lj) == md.batch_key(table) != md.batch_key(wide)
     batch = md.SimulationBatch([lj])
-    assert len(batch) == 1 and batch.grids[-1] == 1
+    assert len(batch) == 1 and batch.grids[-1] == 1 and batch.grids[12] == 0

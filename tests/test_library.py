@@ -45,7 +45,7 @@ def test_md_batch_argument_errors():
         _native.call("md_batch_prepare", ctypes.byref(batch), members, 1, None)
     with pytest.raises(_native.NativeError, match="not prepared"):
         _native.call("md_batch_step", ctypes.byref(_batch._Batch()), None)
-    assert ctypes.sizeof(_batch._Batch) == 8 * 2 + 4 * 6 + 8 * 4 + 8 * 2 * _batch._KERNELS
+    assert ctypes.sizeof(_batch._Batch) == 8 * 2 + 4 * 6 + 8 * 3 + 8 * 3 * _batch._KERNELS
 
 
 def test_op_schemas_match_the_reference():
