@@ -7,9 +7,10 @@ decided on the device. Synthetic, random-seeded inputs (no datasets).
 
 Prints one JSON line: atom-steps/s over K timed steps (after W untimed ones), the force
 kernel's roofline (algorithmic bytes / its event-timed average duration, against 8 TB/s), the
-batched engine's frames/s against the plain per-system path (256 ragged systems), the serving
-loop's p50/p99 latency and throughput against the plain per-request path, and a CPU baseline
-(the oracle's restatement of the reference path on this host's cores).
+batched engine's frames/s against the plain per-system path (256 ragged systems), batched MD's
+atom-steps/s against the plain per-system step graphs (the serving stream's 14 NVT systems), the
+serving loop's p50/p99 latency and throughput against the plain per-request path, and a CPU
+baseline (the oracle's restatement of the reference path on this host's cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batching]
                     [--no-serving]
@@ -154,6 +155,76 @@ def bench_serving(torch, count=64, qps=40.0):
     }
 
 
+def bench_md_batch(torch, steps=200, warmup=20, plain=True):
+    """The serving stream's MD requests (its 14 NVT systems of 262..205,875 atoms, 736,295 in
+    all, LJ or a 200-knot table, fp32), stepped in one ``md.SimulationBatch`` per kind of
+    potential, each on its own stream, against the plain path: each system alone, its own
+    captured step graphs replayed (``Simulation.run``), one after another. Both from the same
+    frames after ``warmup`` steps; atom-steps/s over ``steps``."""
+    from nabladyn_amd import md, serve, workloads
+
+    device = torch.device("cuda", 0)
+    specs = [s for s in workloads.request_stream(64, 40.0, seed=4) if s["kind"] == "md"]
+
+    def simulations(graphs):
+        out = []
+        for r in serve.make_requests([dict(s, arrival=0.0) for s in specs], device):
+            out.append(md.Simulation(
+                r.model, r.positions, r.velocities, r.masses, r.cell, timestep=r.timestep,
+                skin=r.skin, capacity_multiplier=1.25, images=None, step=0,
+                temperature=r.temperature, thermostat_timescale=r.thermostat_timescale,
+                force_group=8, graphs=graphs))
+        return out
+
+    sims = simulations(graphs=False)
+    atoms = sum(s._num_atoms for s in sims)
+    kinds = {}
+    for s in sims:
+        kinds.setdefault((s._pot.kind, s._pot.num_types == 1), []).append(s)
+    batches = [md.SimulationBatch(group) for group in kinds.values()]
+    streams = [torch.cuda.Stream(device) for _ in batches]
+
+    def advance(count):
+        producer = torch.cuda.current_stream(device)
+        for batch, stream in zip(batches, streams):
+            stream.wait_stream(producer)
+            with torch.cuda.stream(stream):
+                batch.advance(count)
+        for stream in streams:
+            producer.wait_stream(stream)
+
+    advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    advance(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    for s in sims:
+        s.check()
+    out = {
+        "systems": len(sims),
+        "atoms": atoms,
+        "batches": len(batches),
+        "steps": steps,
+        "batched_ms_per_step": t_batch / steps * 1e3,
+        "batched_atom_steps_per_s": atoms * steps / t_batch,
+    }
+    if plain:
+        alone = simulations(graphs=True)
+        for s in alone:
+            s.run(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in alone:
+            s.run(steps)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        out.update(plain_ms_per_step=t_plain / steps * 1e3,
+                   plain_atom_steps_per_s=atoms * steps / t_plain,
+                   speedup_vs_plain=t_plain / t_batch)
+    return out
+
+
 def cpu_baseline(x, cell, steps_per_rebuild):
     """The oracle (the reference's eager torch force path + C cell list) on this host's cores:
     one forced rebuild and one force evaluation plus the Verlet/chain update of the same
@@ -212,11 +283,12 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batching", action="store_true")
     parser.add_argument(
-        "--kernel", choices=("force", "copy"),
-        help="only launch one kernel eagerly, for counter collection: 'force' builds the "
+        "--kernel", choices=("force", "copy", "md-batch"),
+        help="only one measurement, for counter collection and traces: 'force' builds the "
              "2M-atom system, runs 100 steps, then launches its force kernel 23 times (the last "
              "23 dispatches of that kernel); 'copy' copies 1 GiB on the device 20 times (the "
-             "scale of the traffic counters)")
+             "scale of the traffic counters); 'md-batch' steps the serving stream's 14 MD "
+             "systems in batches (20 + 200 steps) and prints that leg alone")
     parser.add_argument("--no-serving", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -240,6 +312,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"kernel": "copy", "bytes_read": source.numel() * 4,
                           "bytes_written": source.numel() * 4, "launches": 20}))
+        return 0
+    if args.kernel == "md-batch":
+        print(json.dumps({"kernel": "md-batch", **bench_md_batch(torch, plain=False)}))
         return 0
     log("building the 2,097,152-atom system ...")
     sim, x, cell = build_simulation(torch)
@@ -339,6 +414,8 @@ def main() -> int:
     if not args.no_batching:
         log("batching: 256 ragged systems ...")
         result["batching"] = bench_batching(torch)
+        log("batched MD: the serving stream's 14 NVT systems ...")
+        result["md_batching"] = bench_md_batch(torch)
     if not args.no_serving:
         log("serving: a Poisson request stream ...")
         result["serving"] = bench_serving(torch)

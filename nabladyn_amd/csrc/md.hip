@@ -213,10 +213,15 @@ struct TypedTable
     }
 };
 
-// A table with one type: each block first writes every segment's cubic, (u1, a, b, c) of
-// Table::eval, into its LDS (after the halo and the brick's bookkeeping), so a pair reads one
-// aligned 16-byte (fp64: 32-byte) entry instead of gathering four knots; r and 1/r come from one
-// reciprocal square root instead of a square root and a division.
+// A table with one type: each block first writes every segment's cubic into its LDS (after the
+// halo and the brick's bookkeeping), as two entries of 4. With t the coordinate within the
+// segment, s = max(t, 0) and a, b, c the Catmull-Rom terms of Table::eval:
+//   force  (A, B, C, u1): -du/dr / h^-1 = 0.5 a + b s + 1.5 c s^2, so fs = (A + s (B + s C)) / r
+//          with A = -0.5 a h^-1, B = -b h^-1, C = -1.5 c h^-1 (below the first knot s = 0 leaves
+//          the tangent's slope, as the reference's);
+//   energy (0.5 a, 0.5 b, 0.5 c, 0): u = u1 + s (0.5 a + s (0.5 b + s 0.5 c)) + 0.5 a min(t, 0).
+// A pair of a kick step (no energy) reads one aligned 16-byte (fp64: 32-byte) entry; r and 1/r
+// come from one reciprocal square root, and the segment from a clamped integer conversion.
 template <typename T>
 struct OneTypeTable
 {
@@ -230,58 +235,67 @@ struct OneTypeTable
     const T* __restrict__ inverse_spacing;
     int knots;
 
-    // Segments of the table: K - 1, the last starting at knot K - 2.
+    // Segments of the table: K - 1, the last starting at knot K - 2; two entries each.
     __host__ __device__ int segments() const { return knots - 1; }
 
     __host__ __device__ std::size_t lds_bytes() const
     {
-        return static_cast<std::size_t>(segments()) * sizeof(V4);
+        return static_cast<std::size_t>(segments()) * 2 * sizeof(V4);
     }
 
     __device__ __forceinline__ void stage(unsigned char* lds) const
     {
         V4* out = reinterpret_cast<V4*>(lds);
+        const T h_inverse = inverse_spacing[0];
         for (int s = threadIdx.x; s < segments(); s += blockDim.x)
         {
             const T u0 = padded[s], u1 = padded[s + 1], u2 = padded[s + 2], u3 = padded[s + 3];
-            V4 k;
-            k.x = u1;
-            k.y = u2 - u0;
-            k.z = T(2) * u0 - T(5) * u1 + T(4) * u2 - u3;
-            k.w = T(3) * (u1 - u2) + u3 - u0;
-            out[s] = k;
+            const T a = u2 - u0;
+            const T b = T(2) * u0 - T(5) * u1 + T(4) * u2 - u3;
+            const T c = T(3) * (u1 - u2) + u3 - u0;
+            V4 force, energy;
+            force.x = -T(0.5) * a * h_inverse;
+            force.y = -b * h_inverse;
+            force.z = -T(1.5) * c * h_inverse;
+            force.w = u1;
+            energy.x = T(0.5) * a;
+            energy.y = T(0.5) * b;
+            energy.z = T(0.5) * c;
+            energy.w = T(0);
+            out[2 * s] = force;
+            out[2 * s + 1] = energy;
         }
     }
 
     struct Bound
     {
         const V4* segments;
-        T start, h_inverse, last;
+        T h_inverse, offset;  // x = r h^-1 - start h^-1
+        T last;               // the last segment, K - 2
 
         __device__ __forceinline__ void eval(T, T, T r2, T& u, T& fs) const
         {
             const T inverse_r = rsqrt(r2);
             const T r = r2 * inverse_r;
-            const T x = (r - start) * h_inverse;
-            T segment = floor(x);
-            segment = segment < T(0) ? T(0) : (segment > last ? last : segment);
-            const T t = x - segment;
-            // in range whatever r2 was (a NaN fails both clamps above)
-            const unsigned index = static_cast<unsigned>(static_cast<int>(segment));
-            const V4 k = segments[index <= static_cast<unsigned>(last) ? index : 0u];
+            const T x = r * h_inverse - offset;
+            // the segment, in range whatever r2 was: the clamps send a NaN to 0, so the
+            // conversion always sees a value in [0, last]
+            T clamped = x > T(0) ? x : T(0);
+            clamped = clamped < last ? clamped : last;
+            const int segment = static_cast<int>(clamped);
+            const T t = x - static_cast<T>(segment);
             const T s = t > T(0) ? t : T(0);
-            const T cubic = k.x + T(0.5) * s * (k.y + s * (k.z + s * k.w));
-            const bool below = t < T(0);
-            u = cubic + (below ? T(0.5) * k.y * t : T(0));
-            const T du_dt =
-                below ? T(0.5) * k.y : T(0.5) * (k.y + s * (T(2) * k.z + T(3) * s * k.w));
-            fs = -du_dt * h_inverse * inverse_r;
+            const V4 k = segments[2 * segment];
+            fs = (k.x + s * (k.y + s * k.z)) * inverse_r;
+            const V4 e = segments[2 * segment + 1];
+            u = k.w + s * (e.x + s * (e.y + s * e.z)) + e.x * (t < T(0) ? t : T(0));
         }
     };
 
     __device__ __forceinline__ Bound bind(const unsigned char* lds) const
     {
-        return Bound{reinterpret_cast<const V4*>(lds), start[0], inverse_spacing[0],
+        const T h_inverse = inverse_spacing[0];
+        return Bound{reinterpret_cast<const V4*>(lds), h_inverse, start[0] * h_inverse,
                      static_cast<T>(knots - 2)};
     }
 };
@@ -2226,7 +2240,7 @@ template <typename T>
 bool stages_table(const ndn_potential* pot, std::int64_t halo_capacity)
 {
     if (pot->kind == NDN_LJ || pot->num_types != 1) return false;
-    const std::size_t segments = static_cast<std::size_t>(pot->knots - 1) *
+    const std::size_t segments = static_cast<std::size_t>(pot->knots - 1) * 2 *
                                  sizeof(typename Vec4<T>::type);
     return brick_block_bytes<T>(halo_capacity) + segments <= max_block_bytes();
 }
@@ -3104,7 +3118,7 @@ int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int3
     // one-type tables are staged when the largest fits beside the batch's largest halo, else
     // gathered like typed tables
     const std::size_t point = dtype == NDN_F32 ? sizeof(float4) : sizeof(double4);
-    const std::size_t staged = static_cast<std::size_t>(knots - 1) * point;
+    const std::size_t staged = static_cast<std::size_t>(knots - 1) * 2 * point;
     const std::size_t brick =
         dtype == NDN_F32 ? brick_block_bytes<float>(halo) : brick_block_bytes<double>(halo);
     const bool stage = brick + staged <= max_block_bytes();
