@@ -127,6 +127,21 @@ __device__ __forceinline__ double reciprocal<double>(double value)
     return 1.0 / value;
 }
 
+// 1/sqrt: the hardware's in fp32 (1 ulp; a float argument to rsqrt would take the fp64 routine),
+// the library's in fp64.
+template <typename T>
+__device__ __forceinline__ T reciprocal_sqrt(T value);
+template <>
+__device__ __forceinline__ float reciprocal_sqrt<float>(float value)
+{
+    return __builtin_amdgcn_rsqf(value);
+}
+template <>
+__device__ __forceinline__ double reciprocal_sqrt<double>(double value)
+{
+    return rsqrt(value);
+}
+
 // The force kernel's potentials. A pair's two types arrive as the w of the two gathered
 // float4s (small integers, exact); bind() runs once per kernel and returns what the pair loop
 // evaluates: eval(type_i, type_j, r2, u, fs) with fs = -(du/dr)/r.
@@ -275,7 +290,7 @@ struct OneTypeTable
 
         __device__ __forceinline__ void eval(T, T, T r2, T& u, T& fs) const
         {
-            const T inverse_r = rsqrt(r2);
+            const T inverse_r = reciprocal_sqrt(r2);
             const T r = r2 * inverse_r;
             const T x = r * h_inverse - offset;
             // the segment, in range whatever r2 was: the clamps send a NaN to 0, so the
