@@ -8,9 +8,10 @@ decided on the device. Synthetic, random-seeded inputs (no datasets).
 Prints one JSON line: atom-steps/s over K timed steps (after W untimed ones), the force
 kernel's roofline (algorithmic bytes / its event-timed average duration, against 8 TB/s), the
 batched engine's frames/s against the plain per-system path (256 ragged systems), batched MD's
-atom-steps/s against the plain per-system step graphs (the serving stream's 14 NVT systems), the
-serving loop's p50/p99 latency and throughput against the plain per-request path, and a CPU
-baseline (the oracle's restatement of the reference path on this host's cores).
+atom-steps/s against the plain per-system step graphs (the serving stream's 14 NVT systems), a
+sum of two potentials' E/F/W on the 2M frame with and without an inner list, the serving loop's
+p50/p99 latency and throughput against the plain per-request path, and a CPU baseline (the
+oracle's restatement of the reference path on this host's cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batching]
                     [--no-serving]
@@ -229,6 +230,58 @@ def bench_md_batch(torch, steps=200, warmup=20, plain=True):
     return out
 
 
+def bench_sum(torch, x, cell, repeats=5):
+    """``potential.sum`` on the 2,097,152-atom argon frame (fp32): LJ (argon, rc 0.9 nm) plus a
+    200-knot table (rc 1.2 nm, the LJ's shape), E/F/W through ``quantity.energy_force_virial``
+    on a zero-skin list with an inner list at 0.9 nm and on the 1.2 nm list alone. Returns the
+    mean time of each over ``repeats`` evaluations (HIP events) and the rate against the
+    algorithmic bytes: each part's 8 B per listed pair within its cutoff plus 24 B per atom (x
+    read, F written), as for an E/F/W evaluation."""
+    from nabladyn_amd import potential, quantity, workloads
+    from nabladyn_amd.neighbors import NeighborList
+
+    device = torch.device("cuda", 0)
+    n = x.shape[0]
+    xd = torch.tensor(x, dtype=torch.float32, device=device)
+    cd = torch.tensor(cell, dtype=torch.float32, device=device)
+    types = torch.zeros(n, dtype=torch.int64, device=device)
+    lj = potential.lennard_jones(
+        atom_types=types, sigma=torch.tensor([workloads.ARGON_SIGMA], device=device),
+        epsilon=torch.tensor([workloads.ARGON_EPSILON], device=device), cutoff=0.9)
+    table = potential.tabulated(
+        atom_types=types,
+        table=torch.tensor(workloads.lj_table()[None], dtype=torch.float32, device=device),
+        start=torch.tensor([0.3], device=device), cutoff=1.2)
+    model = potential.sum(lj, table)
+    efw = quantity.energy_force_virial(model)
+    out = {"atoms": n, "parts": "LJ rc 0.9 nm + 200-knot table rc 1.2 nm", "dtype": "fp32"}
+    for label, inner in (("inner_list", (0.9,)), ("one_list", ())):
+        nl = NeighborList(1.2, skin=0.0, inner_cutoffs=inner)
+        nl.allocate(xd, cd)
+        pairs = nl.tensors()
+        efw(xd, cd, pairs)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(repeats):
+            efw(xd, cd, pairs)
+        end.record()
+        torch.cuda.synchronize()
+        seconds = start.elapsed_time(end) / 1e3 / repeats
+        out[f"{label}_ms"] = seconds * 1e3
+        if inner:
+            listed = [int(pairs.inner[0].index.shape[1]), int(pairs.index.shape[1])]
+            near = nl._inner[0].num_pairs
+            far = nl.num_pairs
+            algorithmic = 8 * (near + far) + 2 * 24 * n
+            out.update(pairs_within=[near, far], algorithmic_bytes=algorithmic,
+                       achieved_GBps=algorithmic / seconds / 1e9,
+                       frac=algorithmic / seconds / HBM_PEAK, columns=listed)
+        del nl, pairs
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(x, cell, steps_per_rebuild):
     """The oracle (the reference's eager torch force path + C cell list) on this host's cores:
     one forced rebuild and one force evaluation plus the Verlet/chain update of the same
@@ -287,12 +340,13 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batching", action="store_true")
     parser.add_argument(
-        "--kernel", choices=("force", "copy", "md-batch"),
+        "--kernel", choices=("force", "copy", "md-batch", "sum"),
         help="only one measurement, for counter collection and traces: 'force' builds the "
              "2M-atom system, runs 100 steps, then launches its force kernel 23 times (the last "
              "23 dispatches of that kernel); 'copy' copies 1 GiB on the device 20 times (the "
              "scale of the traffic counters); 'md-batch' steps the serving stream's 14 MD "
-             "systems in batches (20 + 200 steps) and prints that leg alone")
+             "systems in batches (20 + 200 steps) and prints that leg alone; 'sum' evaluates "
+             "potential.sum on the 2M frame and prints that leg alone")
     parser.add_argument("--no-serving", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -319,6 +373,12 @@ def main() -> int:
         return 0
     if args.kernel == "md-batch":
         print(json.dumps({"kernel": "md-batch", **bench_md_batch(torch, plain=False)}))
+        return 0
+    if args.kernel == "sum":
+        from nabladyn_amd import workloads
+
+        x, cell = workloads.argon_lattice(PER_AXIS, seed=0)
+        print(json.dumps({"kernel": "sum", **bench_sum(torch, x, cell)}))
         return 0
     log("building the 2,097,152-atom system ...")
     sim, x, cell = build_simulation(torch)
@@ -420,6 +480,8 @@ def main() -> int:
         result["batching"] = bench_batching(torch)
         log("batched MD: the serving stream's 14 NVT systems ...")
         result["md_batching"] = bench_md_batch(torch)
+        log("potential.sum on the 2M frame ...")
+        result["sum"] = bench_sum(torch, x, cell)
     if not args.no_serving:
         log("serving: a Poisson request stream ...")
         result["serving"] = bench_serving(torch)

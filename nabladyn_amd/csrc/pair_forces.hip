@@ -1,9 +1,10 @@
 // Pair energy, forces and virial on gfx950, for LJ and tabulated potentials in fp32 and fp64.
 //
 // Two schedules share the potential functors and the exact cutoff mask:
-// - half list [2, C] (any source, the reference's NeighborPairs.index): a lane per column, the
-//   pair force added to both atoms with hardware float atomics. It serves quantity.* and the
-//   plain per-system path.
+// - half list [2, C] (any source, the reference's NeighborPairs.index): a lane per column; the
+//   second atom's force added with hardware float atomics, the first atom's summed across each
+//   run of consecutive columns with the same first atom (the drop-in list writes each atom's
+//   pairs together) and added once per run. It serves quantity.* and the plain per-system path.
 // - full list (sliced ELL, rows of 64 in one wave): a lane per atom, forces summed in registers
 //   with no atomics and no re-reading; used by the MD step and the batched path.
 // Energy and virial are summed per block into fp64 partials, then by one block in a fixed
@@ -73,6 +74,21 @@ __global__ void zero_kernel(T* __restrict__ data, std::int64_t count)
 
 // ---- half-list schedule ---------------------------------------------------------------------
 
+// The sum of `value` over each run of lanes with the same key (a run starts where the key
+// changes), left in the run's last lane; `start` is the run's first lane. Equal keys apart are
+// separate runs.
+template <typename T>
+__device__ __forceinline__ T run_sum(T value, int lane, int start)
+{
+#pragma unroll
+    for (int offset = 1; offset < kWave; offset <<= 1)
+    {
+        const T other = __shfl_up(value, offset, kWave);
+        if (lane - offset >= start) value += other;
+    }
+    return value;
+}
+
 template <typename T, class Pot, bool kForces, bool kVirial>
 __global__ void __launch_bounds__(kBlock) half_list_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell,
@@ -81,46 +97,88 @@ __global__ void __launch_bounds__(kBlock) half_list_kernel(
 {
     __shared__ double shared[16];
     const T lx = cell[0], ly = cell[1], lz = cell[2];
+    const int lane = lane_id();
     T energy = 0;
     T w[6] = {0, 0, 0, 0, 0, 0};
     const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t column = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         column < capacity; column += stride)
+    // the wave walks consecutive columns together (every lane takes part in the run sums); a
+    // list that keeps each atom's pairs in consecutive columns, as the drop-in list does, has
+    // long runs of the same first atom, whose force is summed across the run first and added
+    // with one atomic per run
+    for (std::int64_t first =
+             static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x - lane;
+         first < capacity; first += stride)
     {
-        const index_t i = pairs[column];
-        const index_t j = pairs[capacity + column];
+        const std::int64_t column = first + lane;
+        index_t i = -1, j = -1;
+        if (column < capacity)
+        {
+            i = pairs[column];
+            j = pairs[capacity + column];
+        }
         // self-pairs are padding; an index outside [0, N) is skipped rather than read
-        if (i == j || static_cast<std::uint32_t>(i) >= static_cast<std::uint32_t>(num_atoms) ||
-            static_cast<std::uint32_t>(j) >= static_cast<std::uint32_t>(num_atoms))
-            continue;
-        const T* p = positions + 3LL * i;
-        const T* q = positions + 3LL * j;
-        const T dx = displacement(p[0], q[0], lx);
-        const T dy = displacement(p[1], q[1], ly);
-        const T dz = displacement(p[2], q[2], lz);
-        const T r2 = exact_squared(dx, dy, dz);
-        if (!(r2 < cutoff_squared)) continue;
-        T u, fs;
-        pot.eval(pot.row(i, j), r2, u, fs);
-        energy += u;
-        const T fx = fs * dx, fy = fs * dy, fz = fs * dz;
+        const bool listed = i != j && static_cast<std::uint32_t>(i) < static_cast<std::uint32_t>(num_atoms) &&
+                            static_cast<std::uint32_t>(j) < static_cast<std::uint32_t>(num_atoms);
+        T fx = 0, fy = 0, fz = 0;
+        bool inside = false;
+        if (listed)
+        {
+            const T* p = positions + 3LL * i;
+            const T* q = positions + 3LL * j;
+            const T dx = displacement(p[0], q[0], lx);
+            const T dy = displacement(p[1], q[1], ly);
+            const T dz = displacement(p[2], q[2], lz);
+            const T r2 = exact_squared(dx, dy, dz);
+            inside = r2 < cutoff_squared;
+            if (inside)
+            {
+                T u, fs;
+                pot.eval(pot.row(i, j), r2, u, fs);
+                energy += u;
+                fx = fs * dx;
+                fy = fs * dy;
+                fz = fs * dz;
+                if (kVirial)
+                {
+                    w[0] += dx * fx;
+                    w[1] += dy * fy;
+                    w[2] += dz * fz;
+                    w[3] += dx * fy;
+                    w[4] += dx * fz;
+                    w[5] += dy * fz;
+                }
+            }
+        }
         if (kForces)
         {
-            unsafeAtomicAdd(forces + 3LL * i, fx);
-            unsafeAtomicAdd(forces + 3LL * i + 1, fy);
-            unsafeAtomicAdd(forces + 3LL * i + 2, fz);
-            unsafeAtomicAdd(forces + 3LL * j, -fx);
-            unsafeAtomicAdd(forces + 3LL * j + 1, -fy);
-            unsafeAtomicAdd(forces + 3LL * j + 2, -fz);
-        }
-        if (kVirial)
-        {
-            w[0] += dx * fx;
-            w[1] += dy * fy;
-            w[2] += dz * fz;
-            w[3] += dx * fy;
-            w[4] += dx * fz;
-            w[5] += dy * fz;
+            if (inside)
+            {
+                unsafeAtomicAdd(forces + 3LL * j, -fx);
+                unsafeAtomicAdd(forces + 3LL * j + 1, -fy);
+                unsafeAtomicAdd(forces + 3LL * j + 2, -fz);
+            }
+            // runs of the same first atom (padding and unlisted columns key -1)
+            const index_t key = listed ? i : -1;
+            const index_t before = __shfl_up(key, 1, kWave);
+            const bool head = lane == 0 || before != key;
+            int start = head ? lane : 0;
+#pragma unroll
+            for (int offset = 1; offset < kWave; offset <<= 1)
+            {
+                const int other = __shfl_up(start, offset, kWave);
+                if (lane >= offset && other > start) start = other;
+            }
+            const T sx = run_sum(fx, lane, start);
+            const T sy = run_sum(fy, lane, start);
+            const T sz = run_sum(fz, lane, start);
+            const int next_start = __shfl_down(start, 1, kWave);
+            const bool tail = lane == kWave - 1 || next_start != start;
+            if (tail && key >= 0)
+            {
+                unsafeAtomicAdd(forces + 3LL * key, sx);
+                unsafeAtomicAdd(forces + 3LL * key + 1, sy);
+                unsafeAtomicAdd(forces + 3LL * key + 2, sz);
+            }
         }
     }
     store_partials<kVirial>(static_cast<double>(energy), w, shared, partials);

@@ -1,7 +1,7 @@
 """Energy, forces and virial from one evaluation (``quantity.py``).
 
-Each function binds a model (a LennardJones or Tabulated potential of this package, or its
-``_energy_wrapped``) and returns a function of ``(positions, cell, pairs)``. Where the
+Each function binds a model (a LennardJones or Tabulated potential of this package, a Sum of
+them, or its ``_energy_wrapped``) and returns a function of ``(positions, cell, pairs)``. Where the
 reference differentiates the energy with ``torch.func``, here one fused HIP launch computes
 E, F = -dE/dx and W = -dE/dstrain at zero strain (= sum over pairs of r_ij (x) F_ij) over the
 pairs' columns. Results carry no autograd graph; parameter gradients come from
@@ -27,14 +27,30 @@ def _model_of(energy_fn):
     if isinstance(owner, Potential) and getattr(energy_fn, "__name__", "") == "_energy_wrapped":
         return owner, False
     raise TypeError(
-        "quantity functions take a LennardJones or Tabulated model of nabladyn_amd, "
+        "quantity functions take a LennardJones, Tabulated or Sum model of nabladyn_amd, "
         "or its _energy_wrapped",
     )
 
 
 def _evaluate(model, positions: Tensor, cell: Tensor, pairs: NeighborPairs, *,
               energy=False, forces=False, virial=False):
-    """One fused launch over ``pairs.index``; returns (E, F, W) with None for what was not asked."""
+    """One fused launch over ``pairs.index``; returns (E, F, W) with None for what was not asked.
+    A sum takes one launch per part, each on the shortest list that reaches its cutoff
+    (``potential/_sum/_potential.py:86-99``), and adds the parts' E, F and W in part order."""
+    from .potential import Sum
+    from .potential._sum import own_pairs
+
+    if isinstance(model, Sum):
+        parts = [_evaluate(part, positions, cell, own_pairs(pairs, part.cutoff), energy=energy,
+                           forces=forces, virial=virial) for part in model.parts]
+        e = torch.stack([p[0] for p in parts]).sum() if energy else None
+        f = w = None
+        for p in parts:
+            if forces:
+                f = p[1] if f is None else f + p[1]
+            if virial:
+                w = p[2] if w is None else w + p[2]
+        return e, f, w
     dtype, device = positions.dtype, positions.device
     index = pairs.index
     if index.dtype != torch.int32 or not index.is_contiguous():
