@@ -99,17 +99,29 @@ _BRICKS = ((4, 4, 8), (4, 2, 8), (2, 2, 8), (2, 2, 4), (2, 1, 4), (1, 1, 4), (1,
            (1, 1, 1))
 
 
-def brick_shapes(dims: list[int], num_atoms: int, dtype: torch.dtype):
+def brick_shapes(dims: list[int], num_atoms: int, dtype: torch.dtype, reserve: int = 0):
     """The brick shapes worth trying for this grid, largest first: those whose expected halo
-    (atoms per sub-bin times its sub-bins, with a tenth to spare) fits the LDS budget."""
+    (atoms per sub-bin times its sub-bins, with a tenth to spare) fits the LDS budget less
+    `reserve` positions' worth kept for the potential."""
     per_bin = num_atoms / math.prod(dims)
-    limit = halo_limit(dtype)
+    limit = halo_limit(dtype) - reserve
     shapes = []
     for shape in _BRICKS:
         halo = math.prod(min(b + 4, d) for b, d in zip(shape, dims))
         if 1.1 * halo * per_bin <= limit:
             shapes.append(shape)
     return shapes or [_BRICKS[-1]]
+
+
+def potential_reserve(model) -> int:
+    """Halo positions' worth of LDS the force kernel stages for the model beside the halo: a
+    one-type table's segments, two 4-vectors each (the LDS budget of two blocks per CU holds
+    both)."""
+    from ..potential import Tabulated
+
+    if isinstance(model, Tabulated) and model.table.shape[0] == 1:
+        return 2 * (int(model.table.shape[1]) - 1)
+    return 0
 
 
 def halo_limit(dtype: torch.dtype) -> int:
@@ -131,9 +143,11 @@ class _MDNeighbors:
     """The MD path's neighbor state: the reuse rule's tensors and the full list in slot order."""
 
     def __init__(self, cutoff, skin, multiplier, positions, cell, types, group_hint=None,
-                 inner_skin=None):
+                 inner_skin=None, reserve=0):
         from ..neighbors._neighbor_list import grid_dims
 
+        # LDS kept for the potential beside the halo, in halo positions
+        self.reserve = int(reserve)
         self.cutoff, self.skin, self.multiplier = float(cutoff), float(skin), float(multiplier)
         # the pruned list, when its skin is positive and below the list's own
         self.inner_skin = (float(inner_skin) if inner_skin is not None and 0 < inner_skin < skin
@@ -200,7 +214,7 @@ class _MDNeighbors:
         )
         self.dims = dims
         self.dtype = dtype
-        self.set_bricks(brick_shapes(dims, n, dtype)[0])
+        self.set_bricks(brick_shapes(dims, n, dtype, self.reserve)[0])
 
     @property
     def radius(self) -> float:
@@ -230,7 +244,10 @@ class _MDNeighbors:
         """Room for halos of `needed` atoms with a margin (half the rows' multiplier's), up to
         what LDS holds."""
         margin = 1.0 + 0.5 * (self.multiplier - 1.0)
-        capacity = min(max(1, math.ceil(margin * needed)), halo_limit(self.dtype))
+        limit = halo_limit(self.dtype)
+        # within the budget less the potential's share when the halo fits it (else as LDS allows)
+        room = limit - self.reserve if needed <= limit - self.reserve else limit
+        capacity = min(max(1, math.ceil(margin * needed)), room)
         self.buffers["halo_slots"] = torch.zeros(self.bricks * capacity, dtype=torch.int32,
                                                  device=self.buffers["counts"].device)
         self.native.halo_slots = _native.ptr(self.buffers["halo_slots"])
@@ -437,7 +454,8 @@ class Simulation:
         self._pot = pot
         types = model.atom_types if getattr(model, "_num_types", 1) > 1 else None
         self._neighbors = _MDNeighbors(model.cutoff, skin, capacity_multiplier, self.positions,
-                                       cell, types, inner_skin=inner_skin)
+                                       cell, types, inner_skin=inner_skin,
+                                       reserve=potential_reserve(model))
         if types is not None:
             pot.atom_types = _native.ptr(self._neighbors.buffers["types"])
         slots, coef = _native.ctypes.c_int64(), _native.ctypes.c_int64()
@@ -485,7 +503,7 @@ class Simulation:
         """Sort and count (rows and bricks' halos), size them, then fill. Bricks shrink when the
         largest halo would not fit in LDS."""
         nb = self._neighbors
-        shapes = brick_shapes(nb.dims, self._num_atoms, self._dtype)
+        shapes = brick_shapes(nb.dims, self._num_atoms, self._dtype, nb.reserve)
         margin = 1.0 + 0.5 * (nb.multiplier - 1.0)
         for k, shape in enumerate(shapes):
             nb.set_bricks(shape)
@@ -494,7 +512,7 @@ class Simulation:
             self._rebuild(forced=True)
             nb._check_status()
             needed = nb.halo_needed
-            if margin * needed <= halo_limit(self._dtype) or k == len(shapes) - 1:
+            if margin * needed <= halo_limit(self._dtype) - nb.reserve or k == len(shapes) - 1:
                 break
         if needed > halo_limit(self._dtype):
             raise ValueError(
