@@ -178,7 +178,7 @@ def bench_md_batch(torch, steps=200, warmup=20, plain=True):
                 r.model, r.positions, r.velocities, r.masses, r.cell, timestep=r.timestep,
                 skin=r.skin, capacity_multiplier=1.25, images=None, step=0,
                 temperature=r.temperature, thermostat_timescale=r.thermostat_timescale,
-                force_group=8, graphs=graphs))
+                force_group=4, graphs=graphs))
         return out
 
     sims = simulations(graphs=False)

@@ -96,7 +96,7 @@ class Server:
     timing)."""
 
     def __init__(self, device, *, hbm_budget: int = 16 << 30, md_chunk: int = 100,
-                 md_streams: int = 8, efw_schedule=("wave", 2), md_group: int = 8):
+                 md_streams: int = 8, efw_schedule=("wave", 2), md_group: int = 4):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("the server runs on a GPU device")
@@ -287,7 +287,7 @@ def summarize(completed: list[Request], wall: float, **extra) -> dict:
     return out
 
 
-def serve_plain(requests: list[Request], *, md_group: int = 8) -> dict:
+def serve_plain(requests: list[Request], *, md_group: int = 4) -> dict:
     """The per-system baseline: each request alone, in arrival order, on the plain path — a
     zero-skin ``NeighborList`` and ``quantity.energy_force_virial`` for an evaluation,
     ``md.nvt`` and ``run`` for a segment — with the same clock and statistics as ``Server``."""
