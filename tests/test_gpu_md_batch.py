@@ -107,6 +107,46 @@ def test_mixed_potentials_in_one_batch(device):
         assert _same(a, b), k
 
 
+def _typed_system(kind, per, length, device, seed):
+    x = on.jittered_lattice(per, length, 0.02, seed=seed)
+    n = x.shape[0]
+    rng = np.random.default_rng(seed)
+    types = torch.tensor(rng.integers(0, 2, n), dtype=torch.int64, device=device)
+    dtype = torch.float32
+    if kind == "lj":
+        model = potential.lennard_jones(
+            atom_types=types, sigma=torch.tensor([0.34, 0.31, 0.29], dtype=dtype, device=device),
+            epsilon=torch.tensor([0.996, 0.8, 0.6], dtype=dtype, device=device), cutoff=1.0)
+    else:
+        table = np.stack([workloads.lj_table(seed=seed + k) for k in range(3)])
+        model = potential.tabulated(
+            atom_types=types, table=torch.tensor(table, dtype=dtype, device=device),
+            start=torch.full((3,), 0.3, dtype=dtype, device=device), cutoff=1.2)
+    masses = torch.full((n,), MASS, dtype=dtype, device=device)
+    v = md.maxwell_boltzmann(masses, 120.0, seed + 100).to(dtype)
+    args = (model, torch.tensor(x, dtype=dtype, device=device), v, masses,
+            torch.full((3,), length, dtype=dtype, device=device))
+    kw = dict(timestep=0.004, skin=0.2, capacity_multiplier=1.25, images=None, step=0,
+              force_group=4, temperature=120.0, thermostat_timescale=0.1)
+    return lambda graphs=True: md.Simulation(*args, graphs=graphs, **kw)
+
+
+def test_typed_potentials_in_one_batch(device):
+    # two atom types: the typed LJ and typed table force launches in one batch
+    makers = [_typed_system("lj", 8, 2.9, device, 3), _typed_system("table", 9, 3.3, device, 4),
+              _typed_system("lj", 11, 4.0, device, 5), _typed_system("table", 8, 2.9, device, 6)]
+    alone = [make() for make in makers]
+    for s in alone:
+        s.run(80)
+    batched = [make(graphs=False) for make in makers]
+    batch = md.SimulationBatch(batched)
+    assert batch.grids[10] > 0 and batch.grids[11] > 0  # typed LJ, typed table forces
+    batch.advance(80)
+    batch.check()
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        assert _same(a, b), k
+
+
 def test_batch_refuses_mixed_and_empty(device):
     lj = _members("lj", torch.float32, device)[0](graphs=False)
     wide = _members("lj", torch.float64, device)[0](graphs=False)
