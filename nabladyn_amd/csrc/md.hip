@@ -1333,10 +1333,10 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
             const int centers = end - first < kWave ? end - first : kWave;
             const V4 mine = frame.place(pos[lane < centers ? first + lane : first]);
             index_t mine_count = 0;
-            for (index_t chunk = 0; chunk < total; chunk += kWave)
-            {
-                const index_t t = chunk + lane;
-                // the range holding t: the last lane whose start is <= t
+            // candidate t of the stencil: its slot, its local index in the brick's halo (past
+            // an overflowing halo's capacity the sentinel; the status reports it), whether t is
+            // one; the range holding t is the last lane whose start is <= t
+            auto candidate = [&](index_t t, index_t& slot, index_t& local, bool& valid) {
                 int r = 0;
 #pragma unroll
                 for (int step = 32; step > 0; step >>= 1)
@@ -1344,12 +1344,28 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
                     const index_t s = __shfl(start, r + step, kWave);
                     if (s <= t) r += step;
                 }
-                const bool valid = t < total;
-                const index_t slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
-                index_t local = slot + __shfl(to_local, r, kWave);
-                // past an overflowing halo's capacity: the sentinel (the status reports it)
+                valid = t < total;
+                slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
+                local = slot + __shfl(to_local, r, kWave);
                 local = local <= a.halo_capacity ? local : 0;
-                const V4 q = frame.place(pos[valid ? slot : first]);
+            };
+            // each chunk's positions are read while the previous chunk is tested
+            index_t next_slot, next_local;
+            bool next_valid;
+            candidate(lane, next_slot, next_local, next_valid);
+            V4 next_raw = pos[next_valid ? next_slot : first];
+            for (index_t chunk = 0; chunk < total; chunk += kWave)
+            {
+                const index_t slot = next_slot, local = next_local;
+                const bool valid = next_valid;
+                const V4 q = frame.place(next_raw);
+                if (chunk + kWave < total)
+                {
+                    candidate(chunk + kWave + lane, next_slot, next_local, next_valid);
+                    next_raw = pos[next_valid ? next_slot : first];
+                }
+                // the look-ahead read stays here: the scheduler would otherwise sink it to its use
+                __builtin_amdgcn_sched_barrier(0);
                 for (int k = 0; k < centers; ++k)
                 {
                     const index_t center = first + k;
