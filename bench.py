@@ -9,9 +9,10 @@ Prints one JSON line: atom-steps/s over K timed steps (after W untimed ones), th
 kernel's roofline (algorithmic bytes / its event-timed average duration, against 8 TB/s), the
 batched engine's frames/s against the plain per-system path (256 ragged systems), batched MD's
 atom-steps/s against the plain per-system step graphs (the serving stream's 14 NVT systems), a
-sum of two potentials' E/F/W on the 2M frame with and without an inner list, the serving loop's
-p50/p99 latency and throughput against the plain per-request path, and a CPU baseline (the
-oracle's restatement of the reference path on this host's cores).
+sum of two potentials' E/F/W on the 2M frame with and without an inner list, the reference's own
+benchmark (LJ NVT at 512 .. 32,768 atoms, ns/day beside its published RTX 5090 figures), the
+serving loop's p50/p99 latency and throughput against the plain per-request path, and a CPU
+baseline (the oracle's restatement of the reference path on this host's cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batching]
                     [--no-serving]
@@ -282,6 +283,49 @@ def bench_sum(torch, x, cell, repeats=5):
     return out
 
 
+# The reference's published LJ NVT figures (1x RTX 5090, fp32, dt 4 fs, skin 0.1 nm, rc 1.0 nm,
+# Nose-Hoover chain of 3 at 120 K; README.md:594-603): atoms -> ns/day.
+REFERENCE_NS_PER_DAY = {512: 9288.0, 4096: 5507.0, 13824: 3632.0, 32768: 2210.0}
+
+
+def bench_published_sizes(torch, steps=2000, warmup=200):
+    """The reference's own benchmark configuration at its four published sizes: liquid-argon LJ
+    NVT, fp32, jittered simple-cubic lattices of 8^3 .. 32^3 atoms at 21 nm^-3, rc 1.0 nm,
+    skin 0.1 nm, 4 fs, chain of 3 at 120 K, each step one graph replay (``md.nvt`` + ``run``).
+    ns/day and atom-steps/s over ``steps`` steps after ``warmup``, beside the reference's figure
+    (another GPU: context, not a like-for-like target)."""
+    from nabladyn_amd import md, potential, workloads
+
+    device = torch.device("cuda", 0)
+    out = []
+    for n, published in REFERENCE_NS_PER_DAY.items():
+        per_axis = round(n ** (1 / 3))
+        x, cell = workloads.argon_lattice(per_axis, seed=0)
+        masses = torch.full((n,), workloads.ARGON_MASS)
+        velocities = md.maxwell_boltzmann(masses, 120.0, 1)
+        model = potential.lennard_jones(
+            atom_types=torch.zeros(n, dtype=torch.int64, device=device),
+            sigma=torch.tensor([workloads.ARGON_SIGMA], device=device),
+            epsilon=torch.tensor([workloads.ARGON_EPSILON], device=device), cutoff=1.0)
+        sim = md.nvt(model, torch.tensor(x, dtype=torch.float32, device=device),
+                     velocities.float().to(device), masses.to(device),
+                     torch.tensor(cell, dtype=torch.float32, device=device), timestep=0.004,
+                     temperature=120.0, thermostat_timescale=0.1, skin=0.1)
+        sim.run(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        torch.cuda.synchronize()
+        seconds = (time.perf_counter() - t0) / steps
+        ns_per_day = 0.004e-3 * 86400 / seconds
+        out.append({"atoms": n, "us_per_step": seconds * 1e6, "ns_per_day": ns_per_day,
+                    "atom_steps_per_s": n / seconds,
+                    "reference_ns_per_day_rtx5090": published,
+                    "ratio_to_reference": ns_per_day / published})
+        del sim
+    return out
+
+
 def cpu_baseline(x, cell, steps_per_rebuild):
     """The oracle (the reference's eager torch force path + C cell list) on this host's cores:
     one forced rebuild and one force evaluation plus the Verlet/chain update of the same
@@ -340,13 +384,14 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batching", action="store_true")
     parser.add_argument(
-        "--kernel", choices=("force", "copy", "md-batch", "sum"),
+        "--kernel", choices=("force", "copy", "md-batch", "sum", "published"),
         help="only one measurement, for counter collection and traces: 'force' builds the "
              "2M-atom system, runs 100 steps, then launches its force kernel 23 times (the last "
              "23 dispatches of that kernel); 'copy' copies 1 GiB on the device 20 times (the "
              "scale of the traffic counters); 'md-batch' steps the serving stream's 14 MD "
              "systems in batches (20 + 200 steps) and prints that leg alone; 'sum' evaluates "
-             "potential.sum on the 2M frame and prints that leg alone")
+             "potential.sum on the 2M frame and prints that leg alone; 'published' runs the "
+             "reference's published sizes and prints that leg alone")
     parser.add_argument("--no-serving", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -373,6 +418,9 @@ def main() -> int:
         return 0
     if args.kernel == "md-batch":
         print(json.dumps({"kernel": "md-batch", **bench_md_batch(torch, plain=False)}))
+        return 0
+    if args.kernel == "published":
+        print(json.dumps({"kernel": "published", "sizes": bench_published_sizes(torch)}))
         return 0
     if args.kernel == "sum":
         from nabladyn_amd import workloads
@@ -482,6 +530,8 @@ def main() -> int:
         result["md_batching"] = bench_md_batch(torch)
         log("potential.sum on the 2M frame ...")
         result["sum"] = bench_sum(torch, x, cell)
+        log("the reference's published sizes (512 .. 32,768 atoms) ...")
+        result["published_sizes"] = bench_published_sizes(torch)
     if not args.no_serving:
         log("serving: a Poisson request stream ...")
         result["serving"] = bench_serving(torch)

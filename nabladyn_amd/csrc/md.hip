@@ -985,9 +985,9 @@ __device__ __forceinline__ void md_halo_body(const StepArgs<T>& a, const Launch 
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
     const int lane = lane_id();
     const index_t dims[3] = {a.nx, a.ny, a.nz};
-    const std::int64_t waves = static_cast<std::int64_t>(launch.blocks) * (kBlock / kWave);
+    const std::int64_t waves = static_cast<std::int64_t>(launch.blocks) * (blockDim.x / kWave);
     for (std::int64_t brick =
-             static_cast<std::int64_t>(launch.block) * (kBlock / kWave) + threadIdx.x / kWave;
+             static_cast<std::int64_t>(launch.block) * (blockDim.x / kWave) + threadIdx.x / kWave;
          brick < a.bricks; brick += waves)
     {
         const BrickBox box = brick_box(static_cast<index_t>(brick), dims, a.brick);
@@ -1142,6 +1142,49 @@ __global__ void __launch_bounds__(kBlock) md_scatter_halo_kernel(StepArgs<T> a,
     md_scatter_halo_body(a, own_launch(), scatter_blocks, forced);
 }
 
+// Systems of at most kSmallBuild atoms: the whole build (assign, scan, scatter with the halo
+// listing, order, unpack, fill) by one block of 1,024 threads, its phases separated by the
+// block's barriers instead of kernel boundaries: one launch on every step where the build takes
+// six. The gate is read once per phase by one thread (a phase may set a status bit, and the
+// phases after it then do nothing, as the launches of a large build do); every phase is the same
+// body the large build runs, so the list is the same bit for bit.
+constexpr std::int64_t kSmallBuild = 2048;
+
+template <typename T>
+__global__ void __launch_bounds__(kListScanThreads) md_build_small_kernel(StepArgs<T> a, T radius,
+                                                                          T cull_squared,
+                                                                          int forced)
+{
+    __shared__ int open;
+    const Launch one{0, 1};
+    auto gate = [&]() {
+        if (threadIdx.x == 0) open = forced || rebuilding(a.rebuild, a.status);
+        __syncthreads();
+        const bool go = open != 0;
+        // before thread 0 reads the gate again
+        __syncthreads();
+        return go;
+    };
+    if (!gate()) return;
+    md_assign_body(a, one, 1);
+    __syncthreads();
+    if (!gate()) return;
+    md_scan_body(a);
+    __syncthreads();
+    if (!gate()) return;
+    md_scatter_body(a, one, 1);
+    md_halo_body(a, one, 1);
+    __syncthreads();
+    if (!gate()) return;
+    md_order_body(a, one, 1);
+    __syncthreads();
+    if (!gate()) return;
+    md_unpack_body(a, one, 1);
+    __syncthreads();
+    if (!gate()) return;
+    md_fill_body(a, one, radius, cull_squared, 1);
+}
+
 // The periodic frame of a brick. Along each axis every position p is replaced by its image nearest
 // the brick, written so that a plain difference of two images is bit for bit the reference's
 // minimum-image displacement (space.py:11-19) of every pair within `reach` of the brick's atoms:
@@ -1235,9 +1278,9 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
     const bool writing = a.stride > 0;
     const index_t stride = static_cast<index_t>(a.stride);
     // one wave per sub-bin, the waves of the grid striding over the sub-bins
-    const std::int64_t waves = static_cast<std::int64_t>(launch.blocks) * (kBlock / kWave);
+    const std::int64_t waves = static_cast<std::int64_t>(launch.blocks) * (blockDim.x / kWave);
     for (std::int64_t bin =
-             static_cast<std::int64_t>(launch.block) * (kBlock / kWave) + threadIdx.x / kWave;
+             static_cast<std::int64_t>(launch.block) * (blockDim.x / kWave) + threadIdx.x / kWave;
          bin < a.bins; bin += waves)
     {
         // the sub-bin is the wave's: its bounds, and every center's row address below, scalar
@@ -2133,6 +2176,15 @@ template <typename T>
 int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream_t stream)
 {
     const std::int64_t n = a.num_atoms;
+    const T r = static_cast<T>(radius);
+    // culling keeps every sub-bin a pair within R could sit in, with room for rounding
+    const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
+    if (n <= kSmallBuild)
+    {
+        md_build_small_kernel<T><<<1, kListScanThreads, 0, stream>>>(a, r, cull * cull, forced);
+        NDN_LAUNCHED("md_build_small");
+        return NDN_OK;
+    }
     md_assign_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_assign");
     md_scan_kernel<T><<<1, kListScanThreads, 0, stream>>>(a, forced);
@@ -2146,9 +2198,6 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     NDN_LAUNCHED("md_order");
     md_unpack_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_unpack");
-    const T r = static_cast<T>(radius);
-    // culling keeps every sub-bin a pair within R could sit in, with room for rounding
-    const T cull = static_cast<T>(radius * (1.0 + 1e-4) + 1e-6);
     md_fill_kernel<T><<<gated_grid(a.bins, kBlock / kWave), kBlock, 0, stream>>>(
         a, r, cull * cull, forced);
     NDN_LAUNCHED("md_fill");

@@ -28,6 +28,8 @@ _CHUNK_STEPS = 25
 # cutoff + INNER_SKIN, refreshed whenever an atom has moved half of it. Only the force kernel's
 # work depends on it (the full list, its skin and its rebuild rule are the reference's).
 INNER_SKIN = 0.08
+# At most this many atoms, a simulation keeps no pruned list unless asked for one.
+_SMALL_SYSTEM = 2048
 FIELDS = ("potential_energy", "kinetic_energy", "total_energy", "temperature", "pressure")
 
 
@@ -376,7 +378,9 @@ class Simulation:
     It owns its frame, a neighbor list at the model's cutoff (with a full list for the fused
     force kernel) and HIP graphs of one step and of 25, which ``run`` replays. With ``graphs``
     False it captures none and launches each step as it comes (for a simulation stepped in a
-    ``SimulationBatch``, which saves the captures).
+    ``SimulationBatch``, which saves the captures). ``inner_skin`` (nm) is the skin of the pruned
+    list the force kernel reads: "auto" takes ``INNER_SKIN`` above 2,048 atoms and none at or
+    below (the force kernel then reads the full rows); None keeps none.
     """
 
     def __init__(self, model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor,
@@ -384,7 +388,7 @@ class Simulation:
                  images: Tensor | None, step: Integral, temperature: Real | None = None,
                  thermostat_timescale: Real | None = None, thermostat: ChainState | None = None,
                  virial_every_step: bool = False, force_group: int | None = None,
-                 inner_skin: Real | None = INNER_SKIN, graphs: bool = True):
+                 inner_skin: Real | None | str = "auto", graphs: bool = True):
         if not isinstance(model, Potential):
             raise TypeError("model must be a Potential")
         num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
@@ -453,6 +457,10 @@ class Simulation:
         pot, self._keep = model._native()
         self._pot = pot
         types = model.atom_types if getattr(model, "_num_types", 1) > 1 else None
+        if inner_skin == "auto":
+            # the pruned list pays for its refreshes above a few thousand atoms; a small system's
+            # step is set by its launches, and the refresh launch is one of them
+            inner_skin = INNER_SKIN if num_atoms > _SMALL_SYSTEM else None
         self._neighbors = _MDNeighbors(model.cutoff, skin, capacity_multiplier, self.positions,
                                        cell, types, inner_skin=inner_skin,
                                        reserve=potential_reserve(model))
