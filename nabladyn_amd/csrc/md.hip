@@ -1530,13 +1530,14 @@ __device__ __forceinline__ index_t group_max(index_t value)
     return value;
 }
 
-// Four entries of a row, read once per step: one 8-byte non-temporal load, so the list streaming
-// through L2 does not evict the positions the halo loads reuse.
+// Four entries of a row: one 8-byte load. A plain load, not a non-temporal one: a row's lanes
+// read 32 bytes of it per pass, and a line kept in L2 serves the row's next passes (non-temporal
+// reads fetched a third more from HBM and ran 7% slower).
 typedef unsigned Pair2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ Pair2 stream_entries(const std::uint16_t* entries)
 {
-    return __builtin_nontemporal_load(reinterpret_cast<const Pair2*>(entries));
+    return *reinterpret_cast<const Pair2*>(entries);
 }
 
 // The pairs of four entries of a row (one lane's 8-byte read): positions from the halo in LDS,
