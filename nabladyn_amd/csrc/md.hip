@@ -1762,29 +1762,44 @@ __device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch
     const int sub = lane % kPruneGroup;
     const std::uint64_t group = ((1ull << kPruneGroup) - 1ull) << (lane - sub);
     const std::uint64_t below = (1ull << lane) - 1ull;
+    const index_t stride = static_cast<index_t>(a.stride);
     constexpr index_t kRowsPerPass = kBrickThreads<T> / kPruneGroup;
+    constexpr index_t kPass = 4 * kPruneGroup;
+    // the row of this lane's first turn; each turn loads the next turn's count and position
+    index_t r = static_cast<index_t>(threadIdx.x) / kPruneGroup;
+    bool live = r < staged.rows;
+    index_t slot = live ? staged.slot_of(r) : 0;
+    index_t listed = live ? a.counts[slot] : 0;
+    V4 p = pos[slot];
     for (index_t first_row = 0; first_row < staged.rows; first_row += kRowsPerPass)
     {
-        const index_t r = first_row + static_cast<index_t>(threadIdx.x) / kPruneGroup;
-        const bool live = r < staged.rows;
-        const index_t slot = live ? staged.slot_of(r) : 0;
-        V4 c = staged.frame.place(pos[slot]);
+        V4 c = staged.frame.place(p);
         if (!live) c.x = c.y = c.z = -kFarAway<T>;
-        const index_t listed = live ? a.counts[slot] : 0;
-        const index_t count = listed < a.stride ? listed : static_cast<index_t>(a.stride);
+        const index_t count = listed < stride ? listed : stride;
         const index_t limit = (count + 31) & ~31;
         const std::int64_t row_start = static_cast<std::int64_t>(slot) * a.stride;
         const std::uint16_t* entries = a.index + row_start;
         std::uint16_t* out_row = a.inner_index + row_start;
-        index_t kept = 0;
-        // every lane of a row takes the same passes (the rank carries across them)
-        for (index_t pass = 0; pass < limit; pass += 4 * kPruneGroup)
+        // the row's first four passes in flight together (a pass past the row reads its first
+        // entries, which are unused)
+        Pair2 raw[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
         {
-            const index_t base = pass + 4 * sub;
-            const bool any = base < limit;
-            const Pair2 raw = stream_entries(entries + (any ? base : 0));
-            const unsigned local[4] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu,
-                                       raw.y >> 16};
+            const index_t base = k * kPass + 4 * sub;
+            raw[k] = stream_entries(entries + (base < limit ? base : 0));
+        }
+        const index_t r_next = r + kRowsPerPass;
+        const bool live_next = r_next < staged.rows;
+        const index_t slot_next = live_next ? staged.slot_of(r_next) : 0;
+        const index_t listed_next = live_next ? a.counts[slot_next] : 0;
+        const V4 p_next = pos[slot_next];
+        index_t kept = 0;
+        // one pass of a row: its 64 entries, kept in (entry of the lane, lane) order
+        auto prune_pass = [&](const Pair2 quad, const index_t pass) {
+            const bool any = pass + 4 * sub < limit;
+            const unsigned local[4] = {quad.x & 0xffffu, quad.x >> 16, quad.y & 0xffffu,
+                                       quad.y >> 16};
 #pragma unroll
             for (int u = 0; u < 4; ++u)
             {
@@ -1811,13 +1826,27 @@ __device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch
                     out_row[kept + __popcll(mask & below)] = static_cast<std::uint16_t>(local[u]);
                 kept += static_cast<index_t>(__popcll(mask));
             }
+        };
+        // every lane of a row takes the same passes (the rank carries across them)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k * kPass < limit) prune_pass(raw[k], k * kPass);
+        for (index_t pass = 4 * kPass; pass < limit; pass += kPass)
+        {
+            const index_t base = pass + 4 * sub;
+            prune_pass(stream_entries(entries + (base < limit ? base : 0)), pass);
         }
         if (live)
         {
-            for (index_t p = kept + sub; p < ((kept + 31) & ~31); p += kPruneGroup) out_row[p] = 0;
+            for (index_t q = kept + sub; q < ((kept + 31) & ~31); q += kPruneGroup) out_row[q] = 0;
             if (sub == 0) a.inner_counts[slot] = kept;
             if (sub < 3) a.prune_positions[3LL * slot + sub] = a.positions[3LL * slot + sub];
         }
+        r = r_next;
+        live = live_next;
+        slot = slot_next;
+        listed = listed_next;
+        p = p_next;
     }
 }
 
