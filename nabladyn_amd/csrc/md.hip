@@ -2032,30 +2032,40 @@ __device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
 {
     __shared__ double column[kFinishThreads];
     // thread t sums partial (t % 8) of blocks t / 8, t / 8 + 128, ...: coalesced 8-byte reads,
-    // unrolled so that several are in flight, each sum in a fixed order
+    // 16 in flight per thread (two rounds of reads at 4,000 blocks), each sum in a fixed order
+    constexpr int kInFlight = 16;
     const int part = threadIdx.x % 8;
     const std::int64_t values = 8LL * blocks;
     double sum = 0.0;
-    std::int64_t i = threadIdx.x;
-    for (; i + 3 * kFinishThreads < values; i += 4 * kFinishThreads)
+    for (std::int64_t i = threadIdx.x; i < values; i += kInFlight * kFinishThreads)
     {
-        const double v0 = partials[i];
-        const double v1 = partials[i + kFinishThreads];
-        const double v2 = partials[i + 2 * kFinishThreads];
-        const double v3 = partials[i + 3 * kFinishThreads];
-        sum += v0;
-        sum += v1;
-        sum += v2;
-        sum += v3;
+        double v[kInFlight];
+#pragma unroll
+        for (int k = 0; k < kInFlight; ++k)
+        {
+            const std::int64_t at = i + static_cast<std::int64_t>(k) * kFinishThreads;
+            v[k] = at < values ? partials[at] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kInFlight; ++k)
+            if (i + static_cast<std::int64_t>(k) * kFinishThreads < values) sum += v[k];
     }
-    for (; i < values; i += kFinishThreads) sum += partials[i];
     column[threadIdx.x] = sum;
     __syncthreads();
-    // then partial k over the threads t % 8 == k, in order, by lane k of the first wave
+    // then partial k over the threads t % 8 == k, in order, by lane k of the first wave (16
+    // columns read before they are added)
+    static_assert(kFinishThreads % (8 * kInFlight) == 0, "whole batches of columns");
     if (threadIdx.x >= kWave) return;
     double total = 0.0;
     if (threadIdx.x < 8)
-        for (int t = part; t < kFinishThreads; t += 8) total += column[t];
+        for (int t = part; t < kFinishThreads; t += 8 * kInFlight)
+        {
+            double c[kInFlight];
+#pragma unroll
+            for (int j = 0; j < kInFlight; ++j) c[j] = column[t + 8 * j];
+#pragma unroll
+            for (int j = 0; j < kInFlight; ++j) total += c[j];
+        }
     double sums[8];
     for (int k = 0; k < 8; ++k) sums[k] = read_lane(total, k);
     if (threadIdx.x != 0) return;
