@@ -62,34 +62,53 @@ __device__ __forceinline__ double chain_half_step(double* kinetic, double* q, do
 #pragma clang fp contract(off)
     const double quarter = 0.25 * dt;
     const double eighth = 0.125 * dt;
+    // the chain in registers: read once, written once (through the pointers, which the compiler
+    // must assume overlap, every read after a write would go back to memory)
+    double pc[3], qc[3], mc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        pc[k] = p[k];
+        qc[k] = q[k];
+        mc[k] = mass[k];
+    }
     double damping[2];
     double K = *kinetic;
     auto driving = [&](int index) {
-        return index == 0 ? 2.0 * K - g * kT : p[index - 1] * p[index - 1] / mass[index - 1] - kT;
+        return index == 0 ? 2.0 * K - g * kT : pc[index - 1] * pc[index - 1] / mc[index - 1] - kT;
     };
+#pragma unroll
     for (int index = 2; index >= 0; --index)
     {
         if (index == 2)
-            p[2] = p[2] + quarter * driving(2);
+            pc[2] = pc[2] + quarter * driving(2);
         else
         {
             const double kick = quarter * driving(index);
-            p[index] = damping[index] * (damping[index] * p[index] + kick);
+            pc[index] = damping[index] * (damping[index] * pc[index] + kick);
         }
-        if (index > 0) damping[index - 1] = exp(-eighth * p[index] / mass[index]);
+        if (index > 0) damping[index - 1] = exp(-eighth * pc[index] / mc[index]);
     }
-    const double factor = exp(-0.5 * dt * p[0] / mass[0]);
+    const double factor = exp(-0.5 * dt * pc[0] / mc[0]);
     K = K * (factor * factor);
-    for (int k = 0; k < 3; ++k) q[k] = q[k] + 0.5 * dt * p[k] / mass[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qc[k] = qc[k] + 0.5 * dt * pc[k] / mc[k];
+#pragma unroll
     for (int index = 0; index <= 2; ++index)
     {
         if (index == 2)
-            p[2] = p[2] + quarter * driving(2);
+            pc[2] = pc[2] + quarter * driving(2);
         else
         {
             const double kick = quarter * driving(index);
-            p[index] = damping[index] * (damping[index] * p[index] + kick);
+            pc[index] = damping[index] * (damping[index] * pc[index] + kick);
         }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        p[k] = pc[k];
+        q[k] = qc[k];
     }
     *kinetic = K;
     return factor;
