@@ -26,8 +26,10 @@ _STATUS_INTERVAL = 1000
 _CHUNK_STEPS = 25
 # Skin of the pruned list the force kernel reads (nm): the pairs of the full list within
 # cutoff + INNER_SKIN, refreshed whenever an atom has moved half of it. Only the force kernel's
-# work depends on it (the full list, its skin and its rebuild rule are the reference's).
-INNER_SKIN = 0.08
+# work depends on it (the full list, its skin and its rebuild rule are the reference's). At
+# 2,097,152 atoms of argon (cutoff 1.0 nm, skin 0.2 nm) the step took 0.485 / 0.468 / 0.468 /
+# 0.467 / 0.455 / 0.467 / 0.491 ms at 0.06 / 0.07 / 0.08 / 0.09 / 0.10 / 0.12 / 0.14 nm.
+INNER_SKIN = 0.10
 # At most this many atoms, a simulation keeps no pruned list unless asked for one.
 _SMALL_SYSTEM = 2048
 FIELDS = ("potential_energy", "kinetic_energy", "total_energy", "temperature", "pressure")
