@@ -135,6 +135,44 @@ int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num
                  void* forces, void* virial, void* workspace, int64_t workspace_bytes,
                  ndn_stream_t stream);
 
+/* ndn_pair_efw over a full list (every pair within the list radius in both orientations, as
+ * ndn_fill_first_frame / ndn_update write it with full_* buffers): row r is atom rows[r] (rows
+ * NULL: atom r), its k-th neighbor the atom id index[slices[r / 64] + r % 64 + 64 k] for
+ * k < counts[r]; `capacity` entries of index. A lane per row sums the row's force in registers
+ * and writes it once (no atomics); E and W take half of each of a pair's two meetings. Same
+ * results as ndn_pair_efw on the list's half pairs, to rounding. Replaces
+ * quantity.energy_force_virial (quantity.py:93-117) for LJ and tabulated models when the list
+ * keeps a full list. The workspace is ndn_pair_workspace_bytes. */
+int ndn_pair_efw_full(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                      const int32_t* rows, const int32_t* counts, const int32_t* slices,
+                      const int32_t* index, int64_t capacity, const ndn_potential* pot,
+                      void* energy, void* forces, void* virial, void* workspace,
+                      int64_t workspace_bytes, ndn_stream_t stream);
+
+/* ndn_pair_efw at a strain [3, 3] (T, row-major, device memory): each pair's minimum-image
+ * vector d is deformed to (I + strain) d as space.strained does (space.py:51-64), and
+ * energy = E(strain), forces = -dE/dx, virial = -dE/dstrain at that strain (not symmetric in
+ * general; at zero strain it is ndn_pair_efw's). Replaces the strained call of
+ * Potential.forward (potential/_base.py:82-106) differentiated by torch.func, as quantity.py:21-32
+ * sets it up. The workspace is ndn_pair_workspace_bytes. */
+int ndn_pair_efw_strained(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                          const int32_t* pairs, int64_t capacity, const ndn_potential* pot,
+                          const void* strain, void* energy, void* forces, void* virial,
+                          void* workspace, int64_t workspace_bytes, ndn_stream_t stream);
+
+/* dE/dparameters over the half list in fp64, at `strain` [3, 3] (device, may be NULL for
+ * none): LJ out [2P] = dE/dsigma then dE/depsilon per type-pair row; a table out [P, K] =
+ * dE/dtable (the last knot's is 0). Replaces autograd of LennardJones._energy_wrapped
+ * (potential/_lennard_jones/_potential.py:209-234) and Tabulated._energy_wrapped
+ * (potential/_tabulated/_potential.py:146-162) with respect to their parameters. The workspace
+ * is ndn_pair_workspace_bytes. */
+int ndn_pair_parameter_gradient(int dtype, const void* positions, const void* cell,
+                                int64_t num_atoms, const int32_t* pairs, int64_t capacity,
+                                const ndn_potential* pot, const void* strain, double* out,
+                                int64_t out_count, void* workspace, int64_t workspace_bytes,
+                                ndn_stream_t stream);
+
+
 /* ---- molecular dynamics ----------------------------------------------------------------- */
 
 /* The resident frame of a simulation (md/_state.py:8-21), kept in cell order ("slots"): every
@@ -353,7 +391,7 @@ int ndn_md_to_atom_order(int element_bytes, const void* in, const int32_t* ids, 
  * contiguous range of atoms (atom_start) and of one global bin space (bin_start, its sub-bins
  * dims [3] of widths [3], each at least cutoff / span wide). Its potential is LJ (kinds = NDN_LJ,
  * lj_rows [4] = sigma^2, 4 eps, 24 eps, the energy at the cutoff) or a padded table row
- * (table_of >= 0 into tables [R, K + 2], table_start [R], table_inverse_spacing [R]). The rest
+ * (table_of >= 0 into tables [R, K + 2], table_start [R], table_spacing [R]). The rest
  * is scratch (bin_counts zero between calls) and outputs: forces [N, 3], atom_terms [N, 7],
  * energy [S], virial [S, 3, 3]. Positions may lie outside their cells. No reference
  * counterpart: the reference evaluates one system per call. */
@@ -368,12 +406,14 @@ typedef struct ndn_batch {
     const void* widths;
     const void* cells;
     const void* cutoffs;
+    const void* cutoffs_squared; /* [S] each cutoff squared in double, then rounded to T (the
+                                    reference's `squared < cutoff**2` test) */
     const int32_t* kinds;
     const void* lj_rows;
     const int32_t* table_of;
     const void* tables;
     const void* table_start;
-    const void* table_inverse_spacing;
+    const void* table_spacing;
     int64_t knots;
     int64_t total_bins;
     int32_t span;

@@ -35,8 +35,8 @@ class NativeBatch(_native.ctypes.Structure):
     _fields_ = [
         ("num_systems", _i64), ("num_atoms", _i64), ("positions", _vp), ("system_of", _vp),
         ("atom_start", _vp), ("bin_start", _vp), ("dims", _vp), ("widths", _vp), ("cells", _vp),
-        ("cutoffs", _vp), ("kinds", _vp), ("lj_rows", _vp), ("table_of", _vp), ("tables", _vp),
-        ("table_start", _vp), ("table_inverse_spacing", _vp), ("knots", _i64),
+        ("cutoffs", _vp), ("cutoffs_squared", _vp), ("kinds", _vp), ("lj_rows", _vp), ("table_of", _vp), ("tables", _vp),
+        ("table_start", _vp), ("table_spacing", _vp), ("knots", _i64),
         ("total_bins", _i64), ("span", _i32), ("reserved", _i32), ("bin_of", _vp),
         ("bin_counts", _vp), ("bin_offsets", _vp), ("unordered", _vp), ("order", _vp),
         ("wrapped4", _vp), ("sorted4", _vp), ("scan_temp", _vp), ("scan_temp_bytes", _i64),
@@ -56,13 +56,13 @@ def lj_rows(model: LennardJones, dtype) -> Tensor:
 
 def padded_table(model: Tabulated, dtype) -> tuple[Tensor, Tensor, Tensor]:
     """[K + 2] knots of row 0 with the virtual end knots and the last knot 0, its start and
-    1 / spacing (``_tabulated/_potential.py:79-82``)."""
+    spacing (``_tabulated/_potential.py:79-82``)."""
     v = model.table.detach()[0].to(dtype).clone()
     v[-1] = 0
     before = 3 * v[0] - 3 * v[1] + v[2]
     after = 3 * v[-1] - 3 * v[-2] + v[-3]
     padded = torch.cat([before[None], v, after[None]])
-    return padded, model.start[0].to(dtype), (1.0 / model.spacing[0]).to(dtype)
+    return padded, model.start[0].to(dtype), model.spacing[0].to(dtype)
 
 
 def sub_bin_dims(lengths, cutoff: float, span: int, dtype) -> list[int]:
@@ -125,7 +125,7 @@ class Batch:
 
     def _prepare_potentials(self) -> None:
         dt, dev = self.dtype, self.device
-        kinds, rows, table_of, tables, starts, inverses, cutoffs = [], [], [], [], [], [], []
+        kinds, rows, table_of, tables, starts, spacings, cutoffs = [], [], [], [], [], [], []
         knots = 0
         for model in self.models:
             cutoffs.append(model.cutoff)
@@ -136,23 +136,25 @@ class Batch:
             else:
                 kinds.append(_native.TABLE)
                 rows.append(torch.zeros(4, dtype=dt, device=dev))
-                padded, start, inverse = padded_table(model, dt)
+                padded, start, spacing = padded_table(model, dt)
                 if knots and padded.numel() != knots + 2:
                     raise ValueError("every table of a batch must have the same number of knots")
                 knots = padded.numel() - 2
                 table_of.append(len(tables))
                 tables.append(padded)
                 starts.append(start)
-                inverses.append(inverse)
+                spacings.append(spacing)
         ints = {"dtype": torch.int32, "device": dev}
         self.kinds = torch.tensor(kinds, **ints)
         self.lj = torch.stack(rows).contiguous()
         self.table_of = torch.tensor(table_of, **ints)
         self.tables = torch.stack(tables).contiguous() if tables else torch.zeros((1, 4), dtype=dt, device=dev)
         self.table_start = torch.stack(starts) if starts else torch.zeros(1, dtype=dt, device=dev)
-        self.table_inverse = torch.stack(inverses) if inverses else torch.zeros(1, dtype=dt, device=dev)
+        self.table_spacing = torch.stack(spacings) if spacings else torch.ones(1, dtype=dt, device=dev)
         self.knots = knots
         self.cutoffs = torch.tensor(cutoffs, dtype=dt, device=dev)
+        # squared in double, rounded once to the dtype: the reference's `squared < cutoff**2`
+        self.cutoffs_squared = torch.tensor([c * c for c in cutoffs], dtype=dt, device=dev)
 
     def _layout(self, span: int) -> _Layout:
         if span in self._layouts:
@@ -185,9 +187,10 @@ class Batch:
             num_systems=len(self.sizes), num_atoms=n, positions=p(self.positions),
             system_of=p(self.system_of), atom_start=p(self.atom_start), bin_start=p(bin_start),
             dims=p(keep["dims"]), widths=p(widths), cells=p(self.cells), cutoffs=p(self.cutoffs),
+            cutoffs_squared=p(self.cutoffs_squared),
             kinds=p(self.kinds), lj_rows=p(self.lj), table_of=p(self.table_of),
             tables=p(self.tables) if self.knots else None, table_start=p(self.table_start),
-            table_inverse_spacing=p(self.table_inverse), knots=self.knots, total_bins=total,
+            table_spacing=p(self.table_spacing), knots=self.knots, total_bins=total,
             span=span, bin_of=p(keep["bin_of"]), bin_counts=p(keep["bin_counts"]),
             bin_offsets=p(keep["bin_offsets"]), unordered=p(keep["unordered"]),
             order=p(keep["order"]), wrapped4=p(keep["wrapped4"]), sorted4=p(keep["sorted4"]),

@@ -39,12 +39,13 @@ struct BatchArgs
     const T* widths;               // [S, 3]
     const T* cells;                // [S, 3]
     const T* cutoffs;              // [S]
+    const T* cutoffs_squared;      // [S]
     const index_t* kinds;          // [S]
     const LJRow<T>* lj;            // [S]
     const index_t* table_of;       // [S] row of the padded tables, or -1
     const T* tables;               // [R, K + 2]
     const T* table_start;          // [R]
-    const T* table_inverse_spacing;  // [R]
+    const T* table_spacing;  // [R]
     int knots;
     std::int64_t total_bins;
     int span;                      // sub-bins a stencil reaches on each side: 1 or 2
@@ -154,7 +155,7 @@ struct SystemView
     int kind;
     LJRow<T> lj;
     const T* table;
-    T table_start, table_inverse_spacing;
+    T table_start, table_spacing;
 };
 
 template <typename T>
@@ -171,14 +172,14 @@ __device__ __forceinline__ SystemView<T> system_view(const BatchArgs<T>& a, inde
     v.ly = a.cells[3 * s + 1];
     v.lz = a.cells[3 * s + 2];
     v.cutoff = a.cutoffs[s];
-    v.cutoff_squared = v.cutoff * v.cutoff;
+    v.cutoff_squared = a.cutoffs_squared[s];
     v.bin_start = a.bin_start[s];
     v.kind = a.kinds[s];
     v.lj = a.lj[s];
     const index_t row = a.table_of[s];
     v.table = row >= 0 ? a.tables + static_cast<std::int64_t>(row) * (a.knots + 2) : nullptr;
     v.table_start = row >= 0 ? a.table_start[row] : T(0);
-    v.table_inverse_spacing = row >= 0 ? a.table_inverse_spacing[row] : T(0);
+    v.table_spacing = row >= 0 ? a.table_spacing[row] : T(1);
     return v;
 }
 
@@ -195,8 +196,9 @@ __device__ __forceinline__ void pair_terms(const SystemView<T>& v, int knots, T 
         fs = v.lj.twentyfour_eps * (T(2) * p6 * p6 - p6) * inverse;
         return;
     }
+    // r and the knot coordinate with the reference's operations, each correctly rounded
     const T r = sqrt(r2);
-    const T x = (r - v.table_start) * v.table_inverse_spacing;
+    const T x = (r - v.table_start) / v.table_spacing;
     T segment = floor(x);
     segment = segment < T(0) ? T(0) : (segment > T(knots - 2) ? T(knots - 2) : segment);
     const T t = x - segment;
@@ -209,7 +211,7 @@ __device__ __forceinline__ void pair_terms(const SystemView<T>& v, int knots, T 
     const bool below = t < T(0);
     u = u1 + T(0.5) * s * (a1 + s * (b1 + s * c1)) + (below ? T(0.5) * a1 * t : T(0));
     const T du_dt = below ? T(0.5) * a1 : T(0.5) * (a1 + s * (T(2) * b1 + T(3) * s * c1));
-    fs = -du_dt * v.table_inverse_spacing / r;
+    fs = -(du_dt / v.table_spacing) / r;
 }
 
 // The stencil of a center: one lane (of 64) per candidate run of x-consecutive sub-bins in a
@@ -487,7 +489,7 @@ __global__ void __launch_bounds__(kBlock) batch_table_vjp_kernel(BatchArgs<T> a,
                     {
                         const index_t j = a.order[other];
                         const T dist = sqrt(r2);
-                        const T x = (dist - v.table_start) * v.table_inverse_spacing;
+                        const T x = (dist - v.table_start) / v.table_spacing;
                         T segment = floor(x);
                         segment = segment < T(0) ? T(0)
                                 : (segment > T(a.knots - 2) ? T(a.knots - 2) : segment);
@@ -527,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) batch_table_vjp_kernel(BatchArgs<T> a,
                             dz * (gw[6] * dx + gw[7] * dy + gw[8] * dz);
                         // dF and dW weights: d(-(du/dr)/r) = -(d(du/dt) / h) / r
                         const T slope_weight =
-                            -(projection + quadratic) * v.table_inverse_spacing / dist;
+                            -((projection + quadratic) / v.table_spacing) / dist;
                         const int base = static_cast<int>(segment);
                         for (int m = 0; m < 4; ++m)
                         {
@@ -640,12 +642,13 @@ BatchArgs<T> batch_args(const ndn_batch& b)
     a.widths = static_cast<const T*>(b.widths);
     a.cells = static_cast<const T*>(b.cells);
     a.cutoffs = static_cast<const T*>(b.cutoffs);
+    a.cutoffs_squared = static_cast<const T*>(b.cutoffs_squared);
     a.kinds = b.kinds;
     a.lj = static_cast<const LJRow<T>*>(b.lj_rows);
     a.table_of = b.table_of;
     a.tables = static_cast<const T*>(b.tables);
     a.table_start = static_cast<const T*>(b.table_start);
-    a.table_inverse_spacing = static_cast<const T*>(b.table_inverse_spacing);
+    a.table_spacing = static_cast<const T*>(b.table_spacing);
     a.knots = static_cast<int>(b.knots);
     a.total_bins = b.total_bins;
     a.span = b.span;
@@ -728,7 +731,7 @@ int ndn_batch_efw(int dtype, const ndn_batch* batch, int schedule, ndn_stream_t 
     NDN_REQUIRE(b.num_systems >= 1 && b.num_atoms >= 1 && b.num_atoms <= INT32_MAX,
                 "a batch needs systems and at most 2**31 - 1 atoms");
     NDN_REQUIRE(b.positions && b.system_of && b.atom_start && b.bin_start && b.dims && b.widths &&
-                    b.cells && b.cutoffs && b.kinds && b.lj_rows && b.table_of && b.bin_of &&
+                    b.cells && b.cutoffs && b.cutoffs_squared && b.kinds && b.lj_rows && b.table_of && b.bin_of &&
                     b.bin_counts && b.bin_offsets && b.unordered && b.order && b.wrapped4 &&
                     b.sorted4 && b.scan_temp && b.forces && b.atom_terms && b.energy &&
                     b.virial && b.status,

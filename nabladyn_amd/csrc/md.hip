@@ -146,21 +146,6 @@ __device__ __forceinline__ double reciprocal<double>(double value)
     return 1.0 / value;
 }
 
-// 1/sqrt: the hardware's in fp32 (1 ulp; a float argument to rsqrt would take the fp64 routine),
-// the library's in fp64.
-template <typename T>
-__device__ __forceinline__ T reciprocal_sqrt(T value);
-template <>
-__device__ __forceinline__ float reciprocal_sqrt<float>(float value)
-{
-    return __builtin_amdgcn_rsqf(value);
-}
-template <>
-__device__ __forceinline__ double reciprocal_sqrt<double>(double value)
-{
-    return rsqrt(value);
-}
-
 // The force kernel's potentials. A pair's two types arrive as the w of the two gathered
 // float4s (small integers, exact); bind() runs once per kernel and returns what the pair loop
 // evaluates: eval(type_i, type_j, r2, u, fs) with fs = -(du/dr)/r.
@@ -250,12 +235,14 @@ struct TypedTable
 // A table with one type: each block first writes every segment's cubic into its LDS (after the
 // halo and the brick's bookkeeping), as two entries of 4. With t the coordinate within the
 // segment, s = max(t, 0) and a, b, c the Catmull-Rom terms of Table::eval:
-//   force  (A, B, C, u1): -du/dr / h^-1 = 0.5 a + b s + 1.5 c s^2, so fs = (A + s (B + s C)) / r
-//          with A = -0.5 a h^-1, B = -b h^-1, C = -1.5 c h^-1 (below the first knot s = 0 leaves
+//   force  (A, B, C, u1): -du/dr = -(0.5 a + b s + 1.5 c s^2) / h, so fs = (A + s (B + s C)) / r
+//          with A = -0.5 a / h, B = -b / h, C = -1.5 c / h (below the first knot s = 0 leaves
 //          the tangent's slope, as the reference's);
 //   energy (0.5 a, 0.5 b, 0.5 c, 0): u = u1 + s (0.5 a + s (0.5 b + s 0.5 c)) + 0.5 a min(t, 0).
-// A pair of a kick step (no energy) reads one aligned 16-byte (fp64: 32-byte) entry; r and 1/r
-// come from one reciprocal square root, and the segment from a clamped integer conversion.
+// A pair of a kick step (no energy) reads one aligned 16-byte (fp64: 32-byte) entry. r and the
+// knot coordinate x = (r - start) / h come from the reference's own operations, each correctly
+// rounded (square root, subtraction, division), so every pair's segment and t carry the
+// reference's bits; 1/r, which only scales the force, is the hardware reciprocal.
 template <typename T>
 struct OneTypeTable
 {
@@ -266,7 +253,7 @@ struct OneTypeTable
 
     const T* __restrict__ padded;  // [K + 2] (Table::padded of row 0)
     const T* __restrict__ start;
-    const T* __restrict__ inverse_spacing;
+    const T* __restrict__ spacing;
     int knots;
 
     // Segments of the table: K - 1, the last starting at knot K - 2; two entries each.
@@ -280,7 +267,7 @@ struct OneTypeTable
     __device__ __forceinline__ void stage(unsigned char* lds) const
     {
         V4* out = reinterpret_cast<V4*>(lds);
-        const T h_inverse = inverse_spacing[0];
+        const T h = spacing[0];
         for (int s = threadIdx.x; s < segments(); s += blockDim.x)
         {
             const T u0 = padded[s], u1 = padded[s + 1], u2 = padded[s + 2], u3 = padded[s + 3];
@@ -288,9 +275,9 @@ struct OneTypeTable
             const T b = T(2) * u0 - T(5) * u1 + T(4) * u2 - u3;
             const T c = T(3) * (u1 - u2) + u3 - u0;
             V4 force, energy;
-            force.x = -T(0.5) * a * h_inverse;
-            force.y = -b * h_inverse;
-            force.z = -T(1.5) * c * h_inverse;
+            force.x = -(T(0.5) * a) / h;
+            force.y = -b / h;
+            force.z = -(T(1.5) * c) / h;
             force.w = u1;
             energy.x = T(0.5) * a;
             energy.y = T(0.5) * b;
@@ -304,14 +291,13 @@ struct OneTypeTable
     struct Bound
     {
         const V4* segments;
-        T h_inverse, offset;  // x = r h^-1 - start h^-1
-        T last;               // the last segment, K - 2
+        T start, h;
+        T last;  // the last segment, K - 2
 
         __device__ __forceinline__ void eval(T, T, T r2, T& u, T& fs) const
         {
-            const T inverse_r = reciprocal_sqrt(r2);
-            const T r = r2 * inverse_r;
-            const T x = r * h_inverse - offset;
+            const T r = sqrt(r2);
+            const T x = (r - start) / h;
             // the segment, in range whatever r2 was: the clamps send a NaN to 0, so the
             // conversion always sees a value in [0, last]
             T clamped = x > T(0) ? x : T(0);
@@ -320,7 +306,7 @@ struct OneTypeTable
             const T t = x - static_cast<T>(segment);
             const T s = t > T(0) ? t : T(0);
             const V4 k = segments[2 * segment];
-            fs = (k.x + s * (k.y + s * k.z)) * inverse_r;
+            fs = (k.x + s * (k.y + s * k.z)) * reciprocal(r);
             const V4 e = segments[2 * segment + 1];
             u = k.w + s * (e.x + s * (e.y + s * e.z)) + e.x * (t < T(0) ? t : T(0));
         }
@@ -328,8 +314,7 @@ struct OneTypeTable
 
     __device__ __forceinline__ Bound bind(const unsigned char* lds) const
     {
-        const T h_inverse = inverse_spacing[0];
-        return Bound{reinterpret_cast<const V4*>(lds), h_inverse, start[0] * h_inverse,
+        return Bound{reinterpret_cast<const V4*>(lds), start[0], spacing[0],
                      static_cast<T>(knots - 2)};
     }
 };
@@ -532,7 +517,7 @@ __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_po
         const int knots = static_cast<int>(pot.knots);
         const std::int64_t width = knots + 2;
         T* padded = static_cast<T*>(coefficients);
-        T* inverse = padded + rows * width;
+        T* spacing_out = padded + rows * width;
         const T* table = static_cast<const T*>(pot.table);
         const T* spacing = static_cast<const T*>(pot.spacing);
         for (std::int64_t i = threadIdx.x; i < rows * width; i += blockDim.x)
@@ -550,7 +535,7 @@ __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_po
                 out = value(column - 1);
             padded[i] = out;
         }
-        for (int row = threadIdx.x; row < rows; row += blockDim.x) inverse[row] = T(1) / spacing[row];
+        for (int row = threadIdx.x; row < rows; row += blockDim.x) spacing_out[row] = spacing[row];
     }
 }
 

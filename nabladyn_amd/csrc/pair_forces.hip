@@ -20,6 +20,7 @@
 
 #include "common.hpp"
 #include "pair_forces.hpp"
+
 #include "potentials.hpp"
 
 namespace ndn {
@@ -39,15 +40,15 @@ __global__ void lj_rows_kernel(const T* __restrict__ sigma, const T* __restrict_
     out[row] = LJRow<T>{s2, T(4) * e, T(24) * e, T(4) * e * (p6 * p6 - p6)};
 }
 
-// [P, K + 2] padded knots and 1 / spacing (potential/_tabulated/_potential.py:79-82).
+// [P, K + 2] padded knots and the spacing (potential/_tabulated/_potential.py:79-82).
 template <typename T>
 __global__ void table_pad_kernel(const T* __restrict__ table, const T* __restrict__ spacing,
                                  int rows, int knots, T* __restrict__ padded,
-                                 T* __restrict__ inverse_spacing)
+                                 T* __restrict__ spacing_out)
 {
     const std::int64_t index = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const std::int64_t width = knots + 2;
-    if (index < rows) inverse_spacing[index] = T(1) / spacing[index];
+    if (index < rows) spacing_out[index] = spacing[index];
     if (index >= rows * width) return;
     const std::int64_t row = index / width;
     const std::int64_t column = index % width;
@@ -184,6 +185,83 @@ __global__ void __launch_bounds__(kBlock) half_list_kernel(
     store_partials<kVirial>(static_cast<double>(energy), w, shared, partials);
 }
 
+// ---- full-row schedule ----------------------------------------------------------------------
+
+// A full list: every pair within the list radius in both orientations, sliced by 64 rows: row
+// r's k-th neighbor (an atom id) at index[slices[r / 64] + r % 64 + 64 k], its count counts[r];
+// row r is atom rows[r] (rows null: atom r). A lane per row walks it, sums the row's force in
+// registers and writes it once: no atomics. Each pair is met twice, so E and W take half of
+// each meeting. A slot at or past the capacity (an overflowed build) and an id outside [0, N)
+// are skipped rather than read.
+template <typename T, class Pot, bool kForces, bool kVirial>
+__global__ void __launch_bounds__(kBlock) full_rows_kernel(
+    const T* __restrict__ positions, const T* __restrict__ cell, const index_t* __restrict__ rows,
+    const index_t* __restrict__ counts, const index_t* __restrict__ slices,
+    const index_t* __restrict__ index, std::int64_t capacity, index_t num_atoms,
+    T cutoff_squared, Pot pot, T* __restrict__ forces, double* __restrict__ partials)
+{
+    __shared__ double shared[16];
+    const T lx = cell[0], ly = cell[1], lz = cell[2];
+    const int lane = lane_id();
+    T energy = 0;
+    T w[6] = {0, 0, 0, 0, 0, 0};
+    const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (blockDim.x / kWave);
+    const std::int64_t slices_total = (static_cast<std::int64_t>(num_atoms) + kWave - 1) / kWave;
+    for (std::int64_t slice =
+             static_cast<std::int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+         slice < slices_total; slice += waves)
+    {
+        const std::int64_t row = slice * kWave + lane;
+        const bool live = row < num_atoms;
+        const index_t i = live ? (rows != nullptr ? rows[row] : static_cast<index_t>(row)) : 0;
+        const index_t count = live ? counts[row] : 0;
+        const std::int64_t base = live ? static_cast<std::int64_t>(slices[slice]) + lane : 0;
+        const bool own = live && static_cast<std::uint32_t>(i) < static_cast<std::uint32_t>(num_atoms);
+        const T* p = positions + 3LL * (own ? i : 0);
+        const T px = p[0], py = p[1], pz = p[2];
+        T fx = 0, fy = 0, fz = 0;
+        for (index_t k = 0; k < count; ++k)
+        {
+            const std::int64_t slot = base + static_cast<std::int64_t>(k) * kWave;
+            if (!own || slot >= capacity) break;
+            const index_t j = index[slot];
+            if (j == i || static_cast<std::uint32_t>(j) >= static_cast<std::uint32_t>(num_atoms))
+                continue;
+            const T* q = positions + 3LL * j;
+            const T dx = displacement(px, q[0], lx);
+            const T dy = displacement(py, q[1], ly);
+            const T dz = displacement(pz, q[2], lz);
+            const T r2 = exact_squared(dx, dy, dz);
+            if (r2 < cutoff_squared)
+            {
+                T u, fs;
+                pot.eval(pot.row(i, j), r2, u, fs);
+                energy += T(0.5) * u;
+                const T gx = fs * dx, gy = fs * dy, gz = fs * dz;
+                fx += gx;
+                fy += gy;
+                fz += gz;
+                if (kVirial)
+                {
+                    w[0] += T(0.5) * dx * gx;
+                    w[1] += T(0.5) * dy * gy;
+                    w[2] += T(0.5) * dz * gz;
+                    w[3] += T(0.5) * dx * gy;
+                    w[4] += T(0.5) * dx * gz;
+                    w[5] += T(0.5) * dy * gz;
+                }
+            }
+        }
+        if (kForces && own)
+        {
+            forces[3LL * i] = fx;
+            forces[3LL * i + 1] = fy;
+            forces[3LL * i + 2] = fz;
+        }
+    }
+    store_partials<kVirial>(static_cast<double>(energy), w, shared, partials);
+}
+
 // ---- final reduction ------------------------------------------------------------------------
 
 // Sums the per-block partials (8 doubles each: energy, then xx yy zz xy xz yz) in a fixed order
@@ -273,13 +351,13 @@ int prepare_potential(const ndn_potential* pot, void* workspace, hipStream_t str
     {
         const int knots = static_cast<int>(pot->knots);
         T* padded = static_cast<T*>(workspace);
-        T* inverse = padded + static_cast<std::int64_t>(rows) * (knots + 2);
+        T* spacing = padded + static_cast<std::int64_t>(rows) * (knots + 2);
         table_pad_kernel<T><<<blocks_for(static_cast<std::int64_t>(rows) * (knots + 2)), kBlock, 0,
                               stream>>>(static_cast<const T*>(pot->table),
                                         static_cast<const T*>(pot->spacing), rows, knots, padded,
-                                        inverse);
+                                        spacing);
         NDN_LAUNCHED("table_pad");
-        view->table = Table<T>{padded, static_cast<const T*>(pot->start), inverse,
+        view->table = Table<T>{padded, static_cast<const T*>(pot->start), spacing,
                                pot->atom_types, pot->num_types, knots};
     }
     return NDN_OK;
@@ -329,6 +407,65 @@ void launch_half(const T* x, const T* cell, const index_t* pairs, std::int64_t c
     else
         half_list_kernel<T, Pot, false, false><<<blocks, kBlock, 0, stream>>>(
             x, cell, pairs, capacity, num_atoms, cutoff_squared, pot, forces, partials);
+}
+
+template <typename T, class Pot>
+void launch_full(const T* x, const T* cell, const index_t* rows, const index_t* counts,
+                 const index_t* slices, const index_t* index, std::int64_t capacity,
+                 index_t num_atoms, T cutoff_squared, const Pot& pot, T* forces, bool virial,
+                 double* partials, int blocks, hipStream_t stream)
+{
+    if (forces && virial)
+        full_rows_kernel<T, Pot, true, true><<<blocks, kBlock, 0, stream>>>(
+            x, cell, rows, counts, slices, index, capacity, num_atoms, cutoff_squared, pot,
+            forces, partials);
+    else if (forces)
+        full_rows_kernel<T, Pot, true, false><<<blocks, kBlock, 0, stream>>>(
+            x, cell, rows, counts, slices, index, capacity, num_atoms, cutoff_squared, pot,
+            forces, partials);
+    else if (virial)
+        full_rows_kernel<T, Pot, false, true><<<blocks, kBlock, 0, stream>>>(
+            x, cell, rows, counts, slices, index, capacity, num_atoms, cutoff_squared, pot,
+            forces, partials);
+    else
+        full_rows_kernel<T, Pot, false, false><<<blocks, kBlock, 0, stream>>>(
+            x, cell, rows, counts, slices, index, capacity, num_atoms, cutoff_squared, pot,
+            forces, partials);
+}
+
+// Blocks of the full-row launch: a wave per 64 rows, at most kHalfBlocks blocks striding.
+int full_blocks(std::int64_t num_atoms)
+{
+    const std::int64_t slices = (num_atoms + kWave - 1) / kWave;
+    const std::int64_t blocks = (slices + kBlock / kWave - 1) / (kBlock / kWave);
+    return static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(blocks, kHalfBlocks)));
+}
+
+template <typename T>
+int pair_efw_full_impl(const void* positions, const void* cell, std::int64_t num_atoms,
+                       const index_t* rows, const index_t* counts, const index_t* slices,
+                       const index_t* index, std::int64_t capacity, const ndn_potential* pot,
+                       void* energy, void* forces, void* virial, void* workspace,
+                       hipStream_t stream)
+{
+    PotentialView<T> view;
+    const int prepared = prepare_potential<T>(pot, workspace, stream, &view);
+    if (prepared != NDN_OK) return prepared;
+    const int blocks = full_blocks(num_atoms);
+    auto* partials = reinterpret_cast<double*>(static_cast<std::uint8_t*>(workspace) +
+                                               coefficient_bytes(pot, sizeof(T)));
+    T* f = static_cast<T*>(forces);
+    const T* x = static_cast<const T*>(positions);
+    const T* L = static_cast<const T*>(cell);
+    const index_t n = static_cast<index_t>(num_atoms);
+    if (view.kind == NDN_LJ)
+        launch_full<T>(x, L, rows, counts, slices, index, capacity, n, view.cutoff_squared,
+                       view.lj, f, virial != nullptr, partials, blocks, stream);
+    else
+        launch_full<T>(x, L, rows, counts, slices, index, capacity, n, view.cutoff_squared,
+                       view.table, f, virial != nullptr, partials, blocks, stream);
+    NDN_LAUNCHED("full_rows");
+    return finish<T>(partials, blocks, energy, virial, 1.0, stream);
 }
 
 template <typename T>
@@ -399,6 +536,29 @@ int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num
                                     forces, virial, workspace, stream);
     return pair_efw_impl<double>(positions, cell, num_atoms, pairs, capacity, pot, energy, forces,
                                  virial, workspace, stream);
+}
+
+int ndn_pair_efw_full(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                      const int32_t* rows, const int32_t* counts, const int32_t* slices,
+                      const int32_t* index, int64_t capacity, const ndn_potential* pot,
+                      void* energy, void* forces, void* virial, void* workspace,
+                      int64_t workspace_bytes, ndn_stream_t stream)
+{
+    const int checked = check_potential(pot, num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    NDN_REQUIRE(positions && cell && counts && slices && index && workspace, "a buffer is null");
+    NDN_REQUIRE(num_atoms >= 1 && num_atoms <= INT32_MAX, "N must satisfy 1 <= N < 2**31");
+    NDN_REQUIRE(capacity >= 1, "the full list must have a positive capacity");
+    std::int64_t needed = 0;
+    const int sized = ndn_pair_workspace_bytes(dtype, pot, 1, &needed);
+    if (sized != NDN_OK) return sized;
+    NDN_REQUIRE(workspace_bytes >= needed, "the workspace is too small");
+    if (dtype == NDN_F32)
+        return pair_efw_full_impl<float>(positions, cell, num_atoms, rows, counts, slices, index,
+                                         capacity, pot, energy, forces, virial, workspace, stream);
+    return pair_efw_full_impl<double>(positions, cell, num_atoms, rows, counts, slices, index,
+                                      capacity, pot, energy, forces, virial, workspace, stream);
 }
 
 }  // extern "C"

@@ -59,7 +59,7 @@ struct Table
 {
     const T* __restrict__ padded;        // [P, K + 2]
     const T* __restrict__ start;         // [P]
-    const T* __restrict__ inverse_spacing;  // [P]
+    const T* __restrict__ spacing;       // [P]
     const std::int64_t* __restrict__ types;
     int num_types;
     int knots;                           // K
@@ -71,9 +71,11 @@ struct Table
 
     __device__ __forceinline__ void eval(int row_index, T r2, T& u, T& fs) const
     {
+        // r and the knot coordinate with the reference's own operations, each correctly
+        // rounded (sqrt, subtract, divide), so t and the segment are the reference's bits
         const T r = sqrt(r2);
-        const T h_inverse = inverse_spacing[row_index];
-        const T x = (r - start[row_index]) * h_inverse;
+        const T h = spacing[row_index];
+        const T x = (r - start[row_index]) / h;
         T segment = floor(x);
         segment = segment < T(0) ? T(0) : (segment > T(knots - 2) ? T(knots - 2) : segment);
         const T t = x - segment;
@@ -88,7 +90,7 @@ struct Table
         const bool below = t < T(0);
         u = cubic + (below ? T(0.5) * a * t : T(0));
         const T du_dt = below ? T(0.5) * a : T(0.5) * (a + s * (T(2) * b + T(3) * s * c));
-        fs = -du_dt * h_inverse / r;
+        fs = -(du_dt / h) / r;
     }
 };
 

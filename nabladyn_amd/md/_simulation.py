@@ -381,8 +381,9 @@ class Simulation:
     force kernel) and HIP graphs of one step and of 25, which ``run`` replays. With ``graphs``
     False it captures none and launches each step as it comes (for a simulation stepped in a
     ``SimulationBatch``, which saves the captures). ``inner_skin`` (nm) is the skin of the pruned
-    list the force kernel reads: "auto" takes ``INNER_SKIN`` above 2,048 atoms and none at or
-    below (the force kernel then reads the full rows); None keeps none.
+    list the force kernel reads: "auto" takes ``INNER_SKIN`` above 2,048 atoms when the list's
+    skin is larger than it, and none otherwise (the force kernel then reads the full rows); None
+    keeps none; a number must lie strictly between 0 and the list's skin, or none is kept.
     """
 
     def __init__(self, model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor,
@@ -460,9 +461,13 @@ class Simulation:
         self._pot = pot
         types = model.atom_types if getattr(model, "_num_types", 1) > 1 else None
         if inner_skin == "auto":
-            # the pruned list pays for its refreshes above a few thousand atoms; a small system's
-            # step is set by its launches, and the refresh launch is one of them
-            inner_skin = INNER_SKIN if num_atoms > _SMALL_SYSTEM else None
+            # The pruned list pays for its refreshes above a few thousand atoms when the list's
+            # skin leaves room for it: at the default skin of 0.1 nm a pruned skin of 0.1 nm would
+            # prune nothing, and one of 0.05 nm saves too few pairs to pay for refreshes every
+            # few steps, so a list skin of 0.1 nm or less keeps none. A small system's step is
+            # set by its launches, and the refresh launch is one of them.
+            inner_skin = (INNER_SKIN if num_atoms > _SMALL_SYSTEM and INNER_SKIN < float(skin)
+                          else None)
         self._neighbors = _MDNeighbors(model.cutoff, skin, capacity_multiplier, self.positions,
                                        cell, types, inner_skin=inner_skin,
                                        reserve=potential_reserve(model))

@@ -1,6 +1,11 @@
-"""The Potential contract (``potential/_base.py``): a model whose one scalar energy defines
-forces and virial. Here every model evaluates through the library's fused HIP kernels:
-``forward`` gives E; ``nabladyn_amd.quantity`` gives E, F and W from one launch.
+"""The Potential contract (``potential/_base.py``): a model of a fixed set of atoms whose one
+scalar energy defines forces and virial.
+
+A model here is an ``nn.Module`` with four properties (cutoff, atom count, dtype, device), an
+energy of wrapped positions (``_energy_wrapped``), and optionally a kernel description
+(``_native``) that lets ``quantity`` and the MD step evaluate it with the library's fused
+kernels. ``forward`` checks its inputs, wraps the positions and returns the energy, which is
+differentiable (positions, strain, parameters), as the reference's is.
 """
 
 from abc import ABC, abstractmethod
@@ -8,32 +13,33 @@ from abc import ABC, abstractmethod
 import torch
 from torch import Tensor, nn
 
-from .. import _native, space
+from .. import space
 from ..neighbors import NeighborPairs
 
 
-def _check_tensor(name, value, shape, dtype, device) -> None:
-    """``potential/_base.py:11-23``."""
+def _mismatch(name, value, shape, dtype, device) -> str | None:
+    """The reference's message for an input of the wrong kind, or None
+    (``potential/_base.py:11-23``)."""
     if not isinstance(value, Tensor) or value.shape != shape:
-        raise ValueError(f"{name} must be a tensor of shape {list(shape)}")
-    if value.dtype != dtype or value.device != device:
-        raise ValueError(f"{name} must be {dtype} on {device} to match the model")
+        return f"{name} must be a tensor of shape {list(shape)}"
+    if (value.dtype, value.device) != (dtype, device):
+        return f"{name} must be {dtype} on {device} to match the model"
+    return None
 
 
-def _check_pairs(pairs: NeighborPairs, num_atoms: int, cutoff: float, device) -> None:
-    """``potential/_base.py:26-43``."""
+def _pairs_mismatch(pairs: NeighborPairs, num_atoms: int, cutoff: float, device) -> str | None:
+    """The reference's message for pairs that do not fit the model, or None
+    (``potential/_base.py:26-43``): every list [2, C] on the model's device and for its atoms,
+    the outer list reaching the cutoff."""
     for listed in (pairs, *pairs.inner):
-        index = listed.index
-        if index.ndim != 2 or index.shape[0] != 2 or index.device != device:
-            raise ValueError(f"pairs.index must be a [2, C] tensor on {device}")
+        shape_ok = listed.index.ndim == 2 and listed.index.shape[0] == 2
+        if not shape_ok or listed.index.device != device:
+            return f"pairs.index must be a [2, C] tensor on {device}"
         if listed.num_atoms != num_atoms:
-            raise ValueError(
-                f"the pairs are for {listed.num_atoms} atoms, the model for {num_atoms}",
-            )
+            return f"the pairs are for {listed.num_atoms} atoms, the model for {num_atoms}"
     if pairs.cutoff < cutoff:
-        raise ValueError(
-            f"the neighbor list's cutoff {pairs.cutoff} nm is below the model's {cutoff} nm",
-        )
+        return f"the neighbor list's cutoff {pairs.cutoff} nm is below the model's {cutoff} nm"
+    return None
 
 
 class Potential(nn.Module, ABC):
@@ -61,40 +67,52 @@ class Potential(nn.Module, ABC):
 
     @property
     def _list_cutoffs(self) -> tuple[float, ...]:
+        """The cutoffs of the lists the model reads, ascending; the last is its own."""
         return (self.cutoff,)
 
-    @abstractmethod
-    def _native(self) -> tuple[_native.Potential, tuple[Tensor, ...]]:
+    @property
+    def _fused(self) -> bool:
+        """Whether the library's fused kernels evaluate the model (``_native`` describes it)."""
+        return False
+
+    def _native(self):
         """The C-ABI description of the model and the tensors it points into."""
+        raise TypeError(f"{type(self).__name__} has no kernel of its own")
 
     def _check_inputs(self, positions, cell, pairs, strain=None) -> None:
-        _check_tensor("positions", positions, (self.num_atoms, 3), self.dtype, self.device)
-        _check_tensor("cell", cell, (3,), self.dtype, self.device)
-        if strain is not None:
-            _check_tensor("strain", strain, (3, 3), self.dtype, self.device)
-        _check_pairs(pairs, self.num_atoms, self.cutoff, self.device)
+        problems = [
+            _mismatch("positions", positions, (self.num_atoms, 3), self.dtype, self.device),
+            _mismatch("cell", cell, (3,), self.dtype, self.device),
+            None if strain is None else _mismatch("strain", strain, (3, 3), self.dtype, self.device),
+            _pairs_mismatch(pairs, self.num_atoms, self.cutoff, self.device),
+        ]
+        for problem in problems:
+            if problem is not None:
+                raise ValueError(problem)
 
     def forward(self, positions: Tensor, cell: Tensor, pairs: NeighborPairs,
                 strain: Tensor | None = None) -> Tensor:
-        """The energy in kJ/mol of positions [N, 3] (anywhere) in the cell [3]. No host sync.
+        """The energy in kJ/mol of positions [N, 3] in nm (anywhere) in the cell [3].
 
-        ``strain`` is accepted only as None or zero: the virial comes from
-        ``quantity.energy_force_virial``, which computes -dE/dstrain at zero strain directly.
+        ``pairs`` must come from a neighbor list updated to these positions and cell, with a
+        cutoff at least the model's. The optional ``strain`` [3, 3] deforms positions and cell
+        together, x -> (I + strain) x. The energy is differentiable with respect to the
+        positions, the strain and the model's parameters. No host sync.
+
+        Raises ``ValueError`` for inputs of another shape, dtype or device, or for pairs of
+        another system or a shorter cutoff.
         """
         self._check_inputs(positions, cell, pairs, strain)
         wrapped, _ = space.wrap(positions, cell)
         return self._energy_wrapped(wrapped, cell, pairs, strain)
 
+    @abstractmethod
     def _energy_wrapped(self, positions: Tensor, cell: Tensor, pairs: NeighborPairs,
                         strain: Tensor | None = None) -> Tensor:
         """``forward``'s energy for positions already in [0, L), without checks."""
-        from .. import quantity
-
-        if strain is not None and bool(strain.any()):
-            raise NotImplementedError("a nonzero strain is not supported; use quantity.virial")
-        return quantity._evaluate(self, positions, cell, pairs, energy=True)[0]
 
     def _trained_masks(self) -> dict[str, Tensor]:
+        """A bool mask per parameter that trains only some of its entries (True: trained)."""
         return {}
 
 

@@ -1,0 +1,177 @@
+"""The energy of an LJ or tabulated model as a differentiable tensor.
+
+``pair_energy`` returns E of a model on a half list from one launch of the library's pair
+kernel, which computes E, F = -dE/dx and W = -dE/dstrain together. Its autograd function hands
+F and W back in the backward pass (-F gE for the positions, -W gE for the strain) and launches
+the parameter-gradient kernel only when a parameter asks for its gradient. So
+``torch.func.grad(model)``, ``torch.autograd.grad(model(...), parameters)`` and the
+``quantity`` functions of a plain callable that calls a model compose as they do over the
+reference's eager energies (``potential/_base.py:82-106``, ``quantity.py:93-117``).
+
+Limits, stated where they apply:
+- first derivatives only: a derivative of the backward's results (forces in a loss) raises;
+  force and virial fitting has its own hand-written backward (``nabladyn_amd.fit``);
+- no derivative with respect to the cell (the reference's is that of its minimum-image
+  shifts); the cell's gradient is None;
+- ``vmap`` over frames runs one launch per frame.
+"""
+
+import torch
+from torch import Tensor
+
+from .. import _native
+from ..space import frames as _frames
+
+
+def _workspace(model, capacity: int, device):
+    pot, keep = model._native()
+    size = _native.ctypes.c_int64()
+    _native.call("pair_workspace_bytes", _native.dtype_tag(model.dtype), _native.ctypes.byref(pot),
+                 _native.ctypes.c_int64(capacity), _native.ctypes.byref(size))
+    return torch.empty(size.value, dtype=torch.uint8, device=device), pot, keep
+
+
+def evaluate(model, positions: Tensor, cell: Tensor, index: Tensor, strain: Tensor | None,
+             energy=True, forces=True, virial=True):
+    """E [], F [N, 3] and W [3, 3] of wrapped positions on the half list ``index`` [2, C] (int32,
+    contiguous), at ``strain`` or none, from one launch; None for what is not asked."""
+    x = positions.detach().contiguous()
+    cell = cell.detach().contiguous()
+    dtype, device = x.dtype, x.device
+    capacity = index.shape[1]
+    workspace, pot, keep = _workspace(model, capacity, device)
+    e = torch.empty((), dtype=dtype, device=device) if energy else None
+    f = torch.empty((x.shape[0], 3), dtype=dtype, device=device) if forces else None
+    w = torch.empty((3, 3), dtype=dtype, device=device) if virial else None
+    common = (_native.dtype_tag(dtype), _native.ptr(x), _native.ptr(cell),
+              _native.ctypes.c_int64(x.shape[0]), _native.ptr(index),
+              _native.ctypes.c_int64(capacity), _native.ctypes.byref(pot))
+    outputs = (_native.ptr(e), _native.ptr(f), _native.ptr(w), _native.ptr(workspace),
+               _native.ctypes.c_int64(workspace.numel()), _native.stream(device))
+    if strain is None:
+        _native.call("pair_efw", *common, *outputs)
+    else:
+        _native.call("pair_efw_strained", *common,
+                     _native.ptr(strain.detach().to(dtype).contiguous()), *outputs)
+    del keep
+    return e, f, w
+
+
+def evaluate_full(model, positions: Tensor, cell: Tensor, full, energy=True, forces=True,
+                  virial=True):
+    """E, F and W over a list's full rows (``NeighborPairs.full``): one launch, each atom's
+    force summed in registers, no atomics; None for what is not asked."""
+    x = positions.detach().contiguous()
+    cell = cell.detach().contiguous()
+    dtype, device = x.dtype, x.device
+    workspace, pot, keep = _workspace(model, 1, device)
+    e = torch.empty((), dtype=dtype, device=device) if energy else None
+    f = torch.empty((x.shape[0], 3), dtype=dtype, device=device) if forces else None
+    w = torch.empty((3, 3), dtype=dtype, device=device) if virial else None
+    _native.call("pair_efw_full", _native.dtype_tag(dtype), _native.ptr(x), _native.ptr(cell),
+                 _native.ctypes.c_int64(x.shape[0]), _native.ptr(full.rows),
+                 _native.ptr(full.counts), _native.ptr(full.slices), _native.ptr(full.index),
+                 _native.ctypes.c_int64(full.index.numel()), _native.ctypes.byref(pot),
+                 _native.ptr(e), _native.ptr(f), _native.ptr(w), _native.ptr(workspace),
+                 _native.ctypes.c_int64(workspace.numel()), _native.stream(device))
+    del keep
+    return e, f, w
+
+
+def parameter_gradient(model, positions: Tensor, cell: Tensor, index: Tensor,
+                       strain: Tensor | None) -> Tensor:
+    """dE/dparameters in fp64: LJ [2, P] (sigma, epsilon), a table [P, K]."""
+    x = positions.detach().contiguous()
+    cell = cell.detach().contiguous()
+    capacity = index.shape[1]
+    workspace, pot, keep = _workspace(model, capacity, x.device)
+    shape = model._parameter_gradient_shape()
+    out = torch.empty(shape, dtype=torch.float64, device=x.device)
+    s = None if strain is None else strain.detach().to(x.dtype).contiguous()
+    _native.call("pair_parameter_gradient", _native.dtype_tag(x.dtype), _native.ptr(x),
+                 _native.ptr(cell), _native.ctypes.c_int64(x.shape[0]), _native.ptr(index),
+                 _native.ctypes.c_int64(capacity), _native.ctypes.byref(pot), _native.ptr(s),
+                 _native.ptr(out), _native.ctypes.c_int64(out.numel()), _native.ptr(workspace),
+                 _native.ctypes.c_int64(workspace.numel()), _native.stream(x.device))
+    del keep
+    return out
+
+
+class _FirstOrderOnly(torch.autograd.Function):
+    """Passes ``value`` through; differentiating it again raises. It marks the backward's
+    results, which are exact first derivatives but carry no second-derivative graph."""
+
+    generate_vmap_rule = True
+
+    @staticmethod
+    def forward(value, *dependencies):
+        return value.clone()
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, grad):
+        raise RuntimeError(
+            "second derivatives of the LJ and tabulated energies are not available (the fused "
+            "kernel returns F and W, not their derivatives); force and virial fitting of a "
+            "table uses nabladyn_amd.fit")
+
+
+class _PairEnergy(torch.autograd.Function):
+    """E from one launch; F and W kept for the backward pass (see the module docstring)."""
+
+    @staticmethod
+    def forward(positions, cell, strain, first, second, model, index):
+        e, f, w = evaluate(model, positions, cell, index, strain)
+        return e, f, w
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        positions, cell, strain, first, second, model, index = inputs
+        _, f, w = output
+        ctx.mark_non_differentiable(f, w)
+        ctx.model, ctx.index, ctx.strained = model, index, strain is not None
+        ctx.save_for_backward(positions, cell, strain if strain is not None else positions.new_empty(0),
+                              first, second if second is not None else positions.new_empty(0), f, w)
+
+    @staticmethod
+    def backward(ctx, grad_energy, grad_forces, grad_virial):
+        positions, cell, strain, first, second, f, w = ctx.saved_tensors
+        needs = ctx.needs_input_grad
+        strain_arg = strain if ctx.strained else None
+        live = [t for t in (positions, first, second) if t.requires_grad]
+        grad_positions = grad_strain = grad_first = grad_second = None
+        if needs[0]:
+            grad_positions = _FirstOrderOnly.apply(-f * grad_energy, *live)
+        if needs[2]:
+            grad_strain = _FirstOrderOnly.apply(-w * grad_energy, *live)
+        if needs[3] or needs[4]:
+            g = parameter_gradient(ctx.model, positions, cell, ctx.index, strain_arg)
+            g = g.to(positions.dtype) * grad_energy
+            parts = ctx.model._split_parameter_gradient(g)
+            if needs[3]:
+                grad_first = _FirstOrderOnly.apply(parts[0], *live)
+            if needs[4]:
+                grad_second = _FirstOrderOnly.apply(parts[1], *live)
+        return grad_positions, None, grad_strain, grad_first, grad_second, None, None
+
+    @staticmethod
+    def vmap(info, in_dims, positions, cell, strain, first, second, model, index):
+        args = (positions, cell, strain, first, second)
+        dims = tuple(in_dims[:5])
+        outs = [_PairEnergy.apply(*frame, model, index) for frame in _frames(info, dims, args)]
+        stacked = tuple(torch.stack([o[k] for o in outs]) for k in range(3))
+        return stacked, (0, 0, 0)
+
+
+def pair_energy(model, positions: Tensor, cell: Tensor, index: Tensor,
+                strain: Tensor | None) -> Tensor:
+    """E of `model` (LJ or tabulated) on wrapped positions and a half list, differentiable with
+    respect to positions, strain and the model's parameters."""
+    if index.dtype != torch.int32 or not index.is_contiguous():
+        index = index.to(torch.int32).contiguous()
+    first, second = model._differentiable_parameters()
+    e, _, _ = _PairEnergy.apply(positions, cell, strain, first, second, model, index)
+    return e

@@ -1,8 +1,10 @@
-"""The sum of models (``potential/_sum/_potential.py``): one energy, the sum of the parts'.
+"""The sum of models (``potential/_sum/_potential.py``, built by ``potential/_api.py:273-293``):
+one energy, the sum of the parts'.
 
-Each part is evaluated by the fused kernel on the shortest given list that reaches its cutoff
-(the reference's ``_own_pairs``, ``potential/_sum/_potential.py:34-43``), so a list per cutoff
-changes the cost, never the result: pairs beyond a part's cutoff count nothing in its kernel.
+A part reads the shortest given list that reaches its own cutoff (``list_for``); pairs beyond a
+part's cutoff count nothing for it, so a list per cutoff changes the cost, never the result.
+A sum of fused models is evaluated by ``quantity`` and the MD step part by part with the fused
+kernels; any other part makes the sum an ordinary differentiable energy.
 """
 
 import torch
@@ -12,12 +14,15 @@ from ..neighbors import NeighborPairs
 from ._base import Potential
 
 
-def _check_matching(models) -> None:
-    """``potential/_sum/_potential.py:10-21``: the parts share atoms, dtype and device."""
+def _signature(model) -> tuple:
+    return (model.num_atoms, model.dtype, model.device)
+
+
+def _require_matching(models) -> None:
+    """The parts share atoms, dtype and device (``potential/_sum/_potential.py:10-21``)."""
     first = models[0]
-    shared = (first.num_atoms, first.dtype, first.device)
     for model in models[1:]:
-        if (model.num_atoms, model.dtype, model.device) != shared:
+        if _signature(model) != _signature(first):
             raise ValueError(
                 f"the models of a sum must match: {first.num_atoms} atoms in {first.dtype} on "
                 f"{first.device}, against {model.num_atoms} in {model.dtype} on "
@@ -25,23 +30,18 @@ def _check_matching(models) -> None:
             )
 
 
-def _check_models(models) -> None:
-    """``potential/_sum/_potential.py:24-31``."""
-    if not all(isinstance(model, Potential) for model in models):
-        raise TypeError("every model of a sum must be a Potential")
-    if len(models) < 2:
-        raise ValueError(f"a sum needs at least two models, not {len(models)}")
-    _check_matching(models)
+def list_for(pairs: NeighborPairs, cutoff: float) -> NeighborPairs:
+    """Of the outer list and its inner lists, the one with the smallest cutoff at least
+    ``cutoff``, carrying the shorter lists as its own inner lists
+    (``potential/_sum/_potential.py:34-43``). The outer list always qualifies."""
+    candidates = sorted((*pairs.inner, pairs), key=lambda listed: listed.cutoff)
+    chosen = next(listed for listed in candidates if listed.cutoff >= cutoff)
+    inner = tuple(listed for listed in candidates if listed.cutoff < chosen.cutoff)
+    return NeighborPairs(chosen.index, chosen.num_atoms, chosen.cutoff, inner, chosen.full)
 
 
-def own_pairs(pairs: NeighborPairs, cutoff: float) -> NeighborPairs:
-    """The first list, ascending, that reaches ``cutoff``, with the shorter ones inside
-    (``potential/_sum/_potential.py:34-43``): ``pairs.inner`` and then ``pairs``, which
-    reaches every part's cutoff."""
-    lists = (*pairs.inner, pairs)
-    own = [listed for listed in lists if listed.cutoff >= cutoff][0]
-    shorter = tuple(listed for listed in lists if listed.cutoff < own.cutoff)
-    return NeighborPairs(own.index, own.num_atoms, own.cutoff, shorter)
+# the former name, kept for callers
+own_pairs = list_for
 
 
 class Sum(Potential):
@@ -49,9 +49,13 @@ class Sum(Potential):
 
     def __init__(self, *models: Potential):
         super().__init__()
-        _check_models(models)
-        self._cutoff = max(model.cutoff for model in models)
+        if any(not isinstance(model, Potential) for model in models):
+            raise TypeError("every model of a sum must be a Potential")
+        if len(models) < 2:
+            raise ValueError(f"a sum needs at least two models, not {len(models)}")
+        _require_matching(models)
         self.parts = nn.ModuleList(models)
+        self._cutoff = max(model.cutoff for model in models)
 
     @property
     def cutoff(self) -> float:
@@ -61,40 +65,39 @@ class Sum(Potential):
     def num_atoms(self) -> int:
         return self.parts[0].num_atoms
 
-    # a part converted alone after the sum was built would mix dtypes or devices; every
-    # reader of dtype and device checks first
+    # a part converted on its own after the sum was built would leave the parts mismatched;
+    # dtype and device are read only through this check
     @property
     def dtype(self) -> torch.dtype:
-        _check_matching(self.parts)
+        _require_matching(self.parts)
         return self.parts[0].dtype
 
     @property
     def device(self) -> torch.device:
-        _check_matching(self.parts)
+        _require_matching(self.parts)
         return self.parts[0].device
 
     @property
     def _list_cutoffs(self) -> tuple[float, ...]:
-        return tuple(sorted({value for part in self.parts for value in part._list_cutoffs}))
+        return tuple(sorted({c for part in self.parts for c in part._list_cutoffs}))
+
+    @property
+    def _fused(self) -> bool:
+        return all(part._fused for part in self.parts)
 
     def _trained_masks(self) -> dict[str, Tensor]:
-        return {
-            f"parts.{index}.{name}": mask
-            for index, part in enumerate(self.parts)
-            for name, mask in part._trained_masks().items()
-        }
-
-    def _native(self):
-        raise TypeError(
-            "a sum has no single kernel description: quantity evaluates its parts one by one; "
-            "the MD step takes one LennardJones or Tabulated model",
-        )
+        masks = {}
+        for k, part in enumerate(self.parts):
+            masks.update({f"parts.{k}.{name}": mask for name, mask in part._trained_masks().items()})
+        return masks
 
     def _energy_wrapped(self, positions: Tensor, cell: Tensor, pairs: NeighborPairs,
                         strain: Tensor | None = None) -> Tensor:
-        energies = [part._energy_wrapped(positions, cell, own_pairs(pairs, part.cutoff), strain)
-                    for part in self.parts]
-        return torch.stack(energies).sum()
+        total = None
+        for part in self.parts:
+            e = part._energy_wrapped(positions, cell, list_for(pairs, part.cutoff), strain)
+            total = e if total is None else total + e
+        return total
 
 
 def sum(*models: Potential) -> Potential:  # noqa: A001 (the reference's name)
@@ -115,4 +118,4 @@ def sum(*models: Potential) -> Potential:  # noqa: A001 (the reference's name)
     return Sum(*models)
 
 
-__all__ = ("Sum", "own_pairs", "sum")
+__all__ = ("Sum", "list_for", "own_pairs", "sum")

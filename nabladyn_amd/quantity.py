@@ -1,11 +1,20 @@
-"""Energy, forces and virial from one evaluation (``quantity.py``).
+"""Energy, forces and virial from one scalar energy (``quantity.py``).
 
-Each function binds a model (a LennardJones or Tabulated potential of this package, a Sum of
-them, or its ``_energy_wrapped``) and returns a function of ``(positions, cell, pairs)``. Where the
-reference differentiates the energy with ``torch.func``, here one fused HIP launch computes
-E, F = -dE/dx and W = -dE/dstrain at zero strain (= sum over pairs of r_ij (x) F_ij) over the
-pairs' columns. Results carry no autograd graph; parameter gradients come from
-``nabladyn_amd.fit``.
+Each function binds an energy — any callable ``energy(positions, cell, pairs, strain=None)``,
+such as a model or its ``_energy_wrapped`` — and returns a function of
+``(positions, cell, pairs)`` computing all of its results from one evaluation:
+
+- E is a scalar in kJ/mol;
+- F = -dE/dpositions is [N, 3] in kJ/mol/nm;
+- W = -dE/dstrain at zero strain is [3, 3] in kJ/mol (no kinetic term); the pressure is
+  (2K + trace W) / (3V).
+
+A model the library's kernels evaluate (LJ, a table, a sum of them) or its ``_energy_wrapped``
+takes one fused launch per part, which computes E, F and W together — over the list's full rows
+when it keeps them (no atomics), else over its half list; its results carry no autograd graph.
+Any other energy is differentiated by ``torch.func`` as the reference does
+(``quantity.py:21-117``); models inside it are differentiable energies, so a user function that
+adds, say, a restraint to an LJ model works unchanged.
 """
 
 from collections.abc import Callable
@@ -13,79 +22,97 @@ from collections.abc import Callable
 import torch
 from torch import Tensor
 
-from . import _native, space
+from . import space
 from .neighbors import NeighborPairs
 
 
-def _model_of(energy_fn):
-    """The Potential behind ``energy_fn`` and whether its positions still need wrapping."""
+def _fused_model(energy_fn):
+    """(model, whether it wraps) when the fused kernels evaluate ``energy_fn``, else None."""
     from .potential import Potential
 
     if isinstance(energy_fn, Potential):
-        return energy_fn, True
+        return (energy_fn, True) if energy_fn._fused else None
     owner = getattr(energy_fn, "__self__", None)
-    if isinstance(owner, Potential) and getattr(energy_fn, "__name__", "") == "_energy_wrapped":
+    if (isinstance(owner, Potential) and owner._fused
+            and getattr(energy_fn, "__name__", "") == "_energy_wrapped"):
         return owner, False
-    raise TypeError(
-        "quantity functions take a LennardJones, Tabulated or Sum model of nabladyn_amd, "
-        "or its _energy_wrapped",
-    )
+    return None
 
 
 def _evaluate(model, positions: Tensor, cell: Tensor, pairs: NeighborPairs, *,
               energy=False, forces=False, virial=False):
-    """One fused launch over ``pairs.index``; returns (E, F, W) with None for what was not asked.
-    A sum takes one launch per part, each on the shortest list that reaches its cutoff
-    (``potential/_sum/_potential.py:86-99``), and adds the parts' E, F and W in part order."""
+    """E, F, W of a fused model on wrapped positions, None for what was not asked: one launch
+    per part, each part on the shortest list that reaches its cutoff, added in part order
+    (``potential/_sum/_potential.py:86-99``)."""
     from .potential import Sum
-    from .potential._sum import own_pairs
+    from .potential._energy import evaluate, evaluate_full
+    from .potential._sum import list_for
 
     if isinstance(model, Sum):
-        parts = [_evaluate(part, positions, cell, own_pairs(pairs, part.cutoff), energy=energy,
-                           forces=forces, virial=virial) for part in model.parts]
-        e = torch.stack([p[0] for p in parts]).sum() if energy else None
-        f = w = None
-        for p in parts:
-            if forces:
-                f = p[1] if f is None else f + p[1]
-            if virial:
-                w = p[2] if w is None else w + p[2]
-        return e, f, w
-    dtype, device = positions.dtype, positions.device
+        results = [_evaluate(part, positions, cell, list_for(pairs, part.cutoff), energy=energy,
+                             forces=forces, virial=virial) for part in model.parts]
+        summed = []
+        for k in range(3):
+            values = [r[k] for r in results]
+            if values[0] is None:
+                summed.append(None)
+            elif k == 0:
+                summed.append(torch.stack(values).sum())
+            else:
+                summed.append(_add(values))
+        return tuple(summed)
+    if pairs.full is not None:
+        return evaluate_full(model, positions, cell, pairs.full, energy, forces, virial)
     index = pairs.index
     if index.dtype != torch.int32 or not index.is_contiguous():
         index = index.to(torch.int32).contiguous()
-    x = positions.detach().contiguous()
-    cell = cell.detach().contiguous()
-    pot, keep = model._native()
-    capacity = index.shape[1]
-    size = _native.ctypes.c_int64()
-    tag = _native.dtype_tag(dtype)
-    _native.call("pair_workspace_bytes", tag, _native.ctypes.byref(pot),
-                 _native.ctypes.c_int64(capacity), _native.ctypes.byref(size))
-    workspace = torch.empty(size.value, dtype=torch.uint8, device=device)
-    e = torch.empty((), dtype=dtype, device=device) if energy else None
-    f = torch.empty((x.shape[0], 3), dtype=dtype, device=device) if forces else None
-    w = torch.empty((3, 3), dtype=dtype, device=device) if virial else None
-    _native.call(
-        "pair_efw", tag, _native.ptr(x), _native.ptr(cell), _native.ctypes.c_int64(x.shape[0]),
-        _native.ptr(index), _native.ctypes.c_int64(capacity), _native.ctypes.byref(pot),
-        _native.ptr(e) if e is not None else None, _native.ptr(f), _native.ptr(w),
-        _native.ptr(workspace), _native.ctypes.c_int64(size.value), _native.stream(device),
-    )
-    del keep
-    return e, f, w
+    return evaluate(model, positions, cell, index, None, energy, forces, virial)
+
+
+def _add(values):
+    total = values[0]
+    for value in values[1:]:
+        total = total + value
+    return total
+
+
+def _with_strain(energy_fn):
+    """``energy_fn`` with the strain as its second argument, for ``torch.func``
+    (``quantity.py:21-32``)."""
+
+    def strained_energy(positions, strain, cell, pairs):
+        return energy_fn(positions, cell, pairs, strain=strain)
+
+    return strained_energy
 
 
 def _bind(energy_fn, energy: bool, forces: bool, virial: bool):
-    model, wraps = _model_of(energy_fn)
+    fused = _fused_model(energy_fn)
+    if fused is not None:
+        model, wraps = fused
+
+        def evaluate(positions: Tensor, cell: Tensor, pairs: NeighborPairs):
+            if wraps:
+                model._check_inputs(positions, cell, pairs)
+                positions, _ = space.wrap(positions.detach(), cell)
+            results = _evaluate(model, positions, cell, pairs, energy=energy, forces=forces,
+                                virial=virial)
+            return tuple(v for v in results if v is not None)
+
+        return evaluate
+
+    # any other energy: the reference's route, one grad_and_value over positions and strain
+    argnums = tuple(k for k, wanted in ((0, forces), (1, virial)) if wanted)
+    transform = torch.func.grad_and_value(_with_strain(energy_fn),
+                                          argnums=argnums if len(argnums) > 1 else argnums[0])
 
     def evaluate(positions: Tensor, cell: Tensor, pairs: NeighborPairs):
-        if wraps:
-            model._check_inputs(positions, cell, pairs)
-            positions, _ = space.wrap(positions, cell)
-        e, f, w = _evaluate(model, positions, cell, pairs, energy=energy, forces=forces, virial=virial)
-        return tuple(v for v in (e, f, w) if v is not None)
+        gradients, value = transform(positions, positions.new_zeros((3, 3)), cell, pairs)
+        if len(argnums) == 1:
+            gradients = (gradients,)
+        out = [value] if energy else []
+        out.extend(-g for g in gradients)
+        return tuple(out)
 
     return evaluate
 
@@ -97,12 +124,12 @@ def force(energy_fn: Callable[..., Tensor]):
 
 
 def energy_and_force(energy_fn: Callable[..., Tensor]):
-    """Bind (E, F)."""
+    """Bind (E, F), the energy in kJ/mol and the forces [N, 3] in kJ/mol/nm."""
     return _bind(energy_fn, True, True, False)
 
 
 def force_virial(energy_fn: Callable[..., Tensor]):
-    """Bind (F, W)."""
+    """Bind (F, W), the forces [N, 3] in kJ/mol/nm and the virial [3, 3] in kJ/mol."""
     return _bind(energy_fn, False, True, True)
 
 
@@ -113,12 +140,18 @@ def virial(energy_fn: Callable[..., Tensor]):
 
 
 def energy_virial(energy_fn: Callable[..., Tensor]):
-    """Bind (E, W)."""
+    """Bind (E, W), the energy in kJ/mol and the virial [3, 3] in kJ/mol."""
     return _bind(energy_fn, True, False, True)
 
 
 def energy_force_virial(energy_fn: Callable[..., Tensor]):
-    """Bind (E, F, W): the energy and virial [3, 3] in kJ/mol, forces [N, 3] in kJ/mol/nm."""
+    """Bind (E, F, W): the energy and virial [3, 3] in kJ/mol, forces [N, 3] in kJ/mol/nm.
+
+    ``energy_fn`` is a model, or any energy with its call signature. The result is a function of
+    ``(positions, cell, pairs)``, with ``pairs`` from a ``NeighborList``, that returns all three
+    from one evaluation. F = -dE/dr in the positions r; W = -dE/de in the strain e, at zero
+    strain, with no kinetic term, so the pressure is (2K + tr W) / (3V).
+    """
     return _bind(energy_fn, True, True, True)
 
 

@@ -12,25 +12,61 @@ from torch import Tensor
 from . import _native
 
 
+def frames(info, in_dims, args):
+    """For a ``vmap`` rule that runs one frame at a time: the arguments sliced along their
+    vmapped dims, one tuple per frame."""
+    return [tuple(a.select(d, k) if d is not None else a for a, d in zip(args, in_dims))
+            for k in range(info.batch_size)]
+
+
+class _Wrap(torch.autograd.Function):
+    """The wrap kernel with the reference's derivatives (``space.py:22-38``): the identity in
+    the positions (the crossings are piecewise constant) and -crossings in the cell."""
+
+    @staticmethod
+    def forward(positions, cell):
+        x = positions.detach().contiguous()
+        out = torch.empty_like(x)
+        crossings = torch.zeros(x.shape, dtype=torch.int64, device=x.device)
+        c = cell.detach().to(x.dtype).contiguous()
+        _native.call(
+            "wrap", _native.dtype_tag(x.dtype), _native.ptr(x), _native.ptr(c),
+            _native.ctypes.c_int64(x.shape[0]), _native.ptr(out), _native.ptr(crossings),
+            _native.stream(x.device),
+        )
+        return out, crossings
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        _, crossings = output
+        ctx.mark_non_differentiable(crossings)
+        ctx.save_for_backward(crossings)
+
+    @staticmethod
+    def backward(ctx, grad_wrapped, grad_crossings):
+        (crossings,) = ctx.saved_tensors
+        grad_cell = None
+        if ctx.needs_input_grad[1]:
+            grad_cell = -(grad_wrapped * crossings.to(grad_wrapped.dtype)).sum(dim=0)
+        return grad_wrapped, grad_cell
+
+    @staticmethod
+    def vmap(info, in_dims, positions, cell):
+        outs = [_Wrap.apply(*frame) for frame in frames(info, in_dims, (positions, cell))]
+        return (torch.stack([o[0] for o in outs]), torch.stack([o[1] for o in outs])), (0, 0)
+
+
 def wrap(positions: Tensor, cell: Tensor) -> tuple[Tensor, Tensor]:
     """Wrap positions [N, 3] into the cell, each coordinate in [0, L).
 
     Returns ``(wrapped, crossings)``, crossings [N, 3] int64 with
-    ``positions == wrapped + crossings * cell`` up to rounding. GPU tensors only.
+    ``positions == wrapped + crossings * cell`` up to rounding. GPU tensors only. Differentiable
+    as the reference's: the identity in the positions, -crossings in the cell.
     """
     _native.require_gpu(positions, "positions")
     if positions.dim() != 2 or positions.shape[-1] != 3 or cell.shape != (3,):
         raise ValueError("wrap takes positions [N, 3] and a cell [3]")
-    x = positions.detach().contiguous()
-    out = torch.empty_like(x)
-    crossings = torch.zeros(x.shape, dtype=torch.int64, device=x.device)
-    cell = cell.detach().to(x.dtype).contiguous()
-    _native.call(
-        "wrap", _native.dtype_tag(x.dtype), _native.ptr(x), _native.ptr(cell),
-        _native.ctypes.c_int64(x.shape[0]), _native.ptr(out), _native.ptr(crossings),
-        _native.stream(x.device),
-    )
-    return out, crossings
+    return _Wrap.apply(positions, cell)
 
 
 def wrap_into(positions: Tensor, cell: Tensor, out: Tensor, images: Tensor | None = None) -> None:

@@ -13,7 +13,8 @@ Two restatements of the reference's pair potentials:
 - ``torch_efw``: the reference's eager torch computation restated (gathers, ``where`` masks,
   ``torch.func.grad_and_value`` over positions and strain), in any dtype, on the CPU. It is the
   "reference autograd" the hand-written kernels and backward are compared with, and the CPU
-  baseline's force path.
+  baseline's force path. Its square root is correctly rounded, as on the reference's CUDA
+  platform (torch's CPU ``sqrt`` is not; see ``_RoundedSqrt``).
 
 Parity status: pinned. ``lj_efw`` reproduces the fp64 labels of ``lj_frames.npz`` (2 types) to
 ~1e-13 relative and the documented argon_512 numbers (E = -2526.17 kJ/mol, max |F| = 386.06,
@@ -201,6 +202,33 @@ def t_lj_energy(positions, strain, cell, index, atom_types, sigma, epsilon, cuto
     return torch.where(valid, energy, 0.0).sum()
 
 
+class _RoundedSqrt(torch.autograd.Function):
+    """The square root rounded once to the dtype, as the reference's platform computes it
+    (``torch.sqrt`` on CUDA is ``sqrtf``/``sqrt``, IEEE correctly rounded). torch's CPU routine is
+    not: on an AVX-512 host about 0.7% of its fp32 and fp64 roots are 1 ulp off, which moves the
+    table's knot coordinate (r - start) / h by an ulp. numpy's ``sqrt`` is the hardware's
+    correctly rounded instruction. The derivative is torch's own, grad / (2 sqrt(x)) in the
+    dtype."""
+
+    @staticmethod
+    def forward(x):
+        return torch.from_numpy(np.sqrt(x.detach().cpu().numpy())).to(x.device)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        ctx.save_for_backward(output)
+
+    @staticmethod
+    def backward(ctx, grad):
+        (root,) = ctx.saved_tensors
+        return grad / (2 * root)
+
+
+def rounded_sqrt(x):
+    """``x.sqrt()`` correctly rounded (see ``_RoundedSqrt``)."""
+    return _RoundedSqrt.apply(x)
+
+
 def t_hermite(r, rows, table, start, spacing):
     """``_hermite_energy`` (``_tabulated/_potential.py:59-91``) restated."""
     num_knots = table.shape[1]
@@ -226,7 +254,7 @@ def t_table_energy(positions, strain, cell, index, atom_types, table, start, spa
     vectors = _t_displacement(positions[first], positions[second], cell)
     squared = _t_strained(vectors, strain).square().sum(dim=-1)
     valid = (first != second) & (squared < cutoff**2)
-    r = torch.where(valid, squared, cutoff**2).sqrt()
+    r = rounded_sqrt(torch.where(valid, squared, cutoff**2))
     num_types = int(round((np.sqrt(8 * table.shape[0] + 1) - 1) / 2))
     rows = _t_pair_index(atom_types[first], atom_types[second], num_types)
     energy = t_hermite(r, rows, table, start, spacing)

@@ -1,11 +1,14 @@
-"""The batched engine: every system's E, F, W against the fp64 oracle and the plain per-system
-path, for every schedule; each system's results identical alone and in the mix.
+"""The batched engine: every system's E, F, W against the oracle and the plain per-system path,
+for every schedule; each system's results identical alone and in the mix.
 
-Tolerance (fp32): max|dF| / max|F_ref| and |dW| / max|W_ref| against fp64 on the same fp32
-positions within max(1e-5, 1.5 x the distance of the reference's own fp32 evaluation — its
-autograd restatement run in fp32 — from the same fp64 values). For tables that distance reaches
-2.5e-5 on these systems (the knot coordinate (r - start) / h loses fp32 bits, and net forces are
-small differences of large pair forces); for LJ it stays below 1e-5. fp64: 1e-12.
+Tolerance. fp32: max|dF| / max|F_ref|, |dW| / max|W_ref| and |dE| / max(1, |E_ref|) within 1e-5
+of the reference's own fp32 route (its autograd evaluation, restated by the oracle, run in fp32
+on the same positions and pairs), with each system's error against an fp64 evaluation printed
+beside it. fp64: 1e-12 against the oracle's closed forms. The pairs come from the oracle's C
+cell list (pinned by brute force in the oracle's own tests).
+
+The full batching workload (256 systems, 64 .. 32,768 atoms, half LJ, half a 200-knot table)
+runs once, fp32, autotuned schedule: every system against the route.
 """
 
 import numpy as np
@@ -15,6 +18,7 @@ import torch
 from nabladyn_amd import batch, potential, workloads
 from oracle import neighbors as on
 from oracle import potentials as op
+from _parity import ROUTE_TOL, lj_route, rel, table_route
 
 pytestmark = pytest.mark.gpu
 
@@ -24,61 +28,76 @@ def _systems(device, dtype, count=24, high=4096, seed=2):
     return batch.make_systems(raw, device, dtype)
 
 
+def _pairs(xw, cell, cutoff):
+    """Unordered pairs within the cutoff, [2, P], from the oracle's cell list."""
+    b = on.build(xw, cell, cutoff)
+    return b.pairs[:, :b.pair_total]
+
+
 def _reference(x, cell, model):
-    """fp64 oracle E, F, W on the system's (rounded) positions."""
-    xn = x.cpu().numpy()
-    cn = cell.cpu().numpy()
-    xw = on.wrap(xn, cn)[0].astype(np.float64)
-    cell64 = cn.astype(np.float64)
-    pairs = on.brute_pairs(xw, cell64, model.cutoff).T
-    if isinstance(model, potential.LennardJones):
-        return op.lj_efw(xw, cell64, pairs, np.zeros(len(xw), np.int64),
-                         model.sigma.detach().cpu().numpy(), model.epsilon.detach().cpu().numpy(),
-                         model.cutoff)
-    return op.table_efw(xw, cell64, pairs, np.zeros(len(xw), np.int64),
-                        model.table.detach().cpu().numpy(), model.start.cpu().numpy(),
-                        model.cutoff, spacing=model.spacing.cpu().numpy())
-
-
-def _reference_fp32(x, cell, model):
-    """The reference's autograd route (``quantity.py:93-117``) evaluated in fp32: F, W."""
+    """(fp64 closed form, the reference's route in the system's dtype) of E, F, W."""
     xw = on.wrap(x.cpu().numpy(), cell.cpu().numpy())[0]
-    pairs = torch.tensor(on.brute_pairs(xw.astype(np.float64), cell.cpu().numpy().astype(np.float64),
-                                        model.cutoff).T).long()
-    types = torch.zeros(len(xw), dtype=torch.int64)
-    f32 = torch.float32
+    cn = cell.cpu().numpy()
+    pairs = _pairs(xw, cn, model.cutoff)
+    types = np.zeros(len(xw), np.int64)
+    x64, c64 = xw.astype(np.float64), cn.astype(np.float64)
     if isinstance(model, potential.LennardJones):
-        sigma, eps = model.sigma.detach().cpu().to(f32), model.epsilon.detach().cpu().to(f32)
-        energy = lambda p, s, c, i: op.t_lj_energy(p, s, c, i, types, sigma, eps, model.cutoff)
+        sigma, eps = model.sigma.detach().cpu().numpy(), model.epsilon.detach().cpu().numpy()
+        fp64 = op.lj_efw(x64, c64, pairs, types, sigma, eps, model.cutoff)
+        route = lj_route(xw, cn, pairs, types, sigma, eps, model.cutoff)
     else:
-        args = (model.table.detach().cpu().to(f32), model.start.cpu().to(f32),
-                model.spacing.cpu().to(f32))
-        energy = lambda p, s, c, i: op.t_table_energy(p, s, c, i, types, *args, model.cutoff)
-    _, forces, virial = op.torch_efw(energy, torch.tensor(xw, dtype=f32),
-                                     cell.cpu().to(f32), pairs)
-    return forces.numpy(), virial.numpy()
+        args = (model.table.detach().cpu().numpy(), model.start.cpu().numpy())
+        spacing = model.spacing.cpu().numpy()
+        fp64 = op.table_efw(x64, c64, pairs, types, *args, model.cutoff, spacing=spacing)
+        route = table_route(xw, cn, pairs, types, *args, spacing, model.cutoff)
+    return fp64, route
 
 
-def _rel(a, b):
-    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-30))
+def _check(k, got, fp64, route, dtype, label):
+    e, f, w = got
+    if dtype == torch.float64:
+        assert rel(f, fp64[1]) <= 1e-12, (label, k)
+        assert rel(w, fp64[2]) <= 1e-12, (label, k)
+        assert abs(e - fp64[0]) <= 1e-12 * max(1.0, abs(fp64[0])), (label, k)
+        return None
+    errors = (rel(f, route[1]), rel(w, route[2]), abs(e - route[0]) / max(1.0, abs(route[0])),
+              rel(f, fp64[1]), rel(w, fp64[2]))
+    assert max(errors[:3]) <= ROUTE_TOL, (label, k, errors)
+    return errors
+
+
+def _evaluate_and_check(systems, b, dtype, label):
+    energy, forces, virial = b.evaluate()
+    label = label.format(chosen=getattr(b, "chosen", None))
+    worst = np.zeros(5)
+    for k, (f, (x, cell, model)) in enumerate(zip(b.split(forces), systems)):
+        fp64, route = _reference(x, cell, model)
+        got = (float(energy[k]), f.cpu().numpy(), virial[k].cpu().numpy())
+        errors = _check(k, got, fp64, route, dtype, label)
+        if errors is not None:
+            worst = np.maximum(worst, errors)
+    if dtype == torch.float32:
+        print(f"{label}: worst over {len(systems)} systems: F vs route {worst[0]:.2e}, W vs route "
+              f"{worst[1]:.2e}, E vs route {worst[2]:.2e}; F vs fp64 {worst[3]:.2e}, W vs fp64 "
+              f"{worst[4]:.2e}")
 
 
 @pytest.mark.parametrize("schedule", batch.SCHEDULES)
-@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float64, 1e-12)])
-def test_batch_matches_oracle(device, schedule, dtype, tol):
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_matches_oracle(device, schedule, dtype):
     systems = _systems(device, dtype, count=12 if dtype == torch.float64 else 24)
     b = batch.Batch(systems, schedule=schedule)
-    energy, forces, virial = b.evaluate()
-    for k, (f, (x, cell, model)) in enumerate(zip(b.split(forces), systems)):
-        e0, f0, w0 = _reference(x, cell, model)
-        tol_f = tol_w = tol
-        if dtype == torch.float32:
-            f32, w32 = _reference_fp32(x, cell, model)
-            tol_f = max(tol, 1.5 * _rel(f32, f0))
-            tol_w = max(tol, 1.5 * _rel(w32, w0))
-        assert _rel(f.cpu().numpy(), f0) <= tol_f, (k, schedule)
-        assert _rel(virial[k].cpu().numpy(), w0) <= tol_w, (k, schedule)
-        assert abs(float(energy[k]) - e0) <= tol * max(1.0, abs(e0)) * 10, (k, schedule)
+    _evaluate_and_check(systems, b, dtype, f"schedule {schedule}")
+
+
+def test_batch_full_workload_256_systems(device):
+    """The batching workload at its stated size: 256 systems of 64 .. 32,768 atoms (seed 2),
+    fp32, the schedule autotuned; every system against the reference's route."""
+    raw = workloads.ragged_systems(256, seed=2, min_length=2.45)
+    systems = batch.make_systems(raw, device)
+    assert max(int(x.shape[0]) for x, _, _ in systems) > 16384
+    b = batch.Batch(systems, schedule="auto")
+    _evaluate_and_check(systems, b, torch.float32, "256 systems, schedule {chosen}")
 
 
 def test_batch_matches_plain_path(device):
@@ -87,8 +106,8 @@ def test_batch_matches_plain_path(device):
     energy, forces, virial = b.evaluate()
     plain = batch.plain_efw(systems)
     for k, (f, (e1, f1, w1)) in enumerate(zip(b.split(forces), plain)):
-        assert _rel(f.cpu(), f1.cpu()) <= 2e-5, k
-        assert _rel(virial[k].cpu(), w1.cpu()) <= 2e-5, k
+        assert rel(f.cpu(), f1.cpu()) <= 2e-5, k
+        assert rel(virial[k].cpu(), w1.cpu()) <= 2e-5, k
 
 
 @pytest.mark.parametrize("schedule", [("wave", 2), ("thread", 1)])

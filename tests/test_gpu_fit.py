@@ -26,6 +26,13 @@ KNOTS, START, CUTOFF = 200, 0.3, 1.2
 WEIGHTS = {"energy": 1.0, "forces": 1.0, "virial": 1.0}
 
 
+def _pairs(xw, cell):
+    """Unordered pairs within the cutoff [2, P] from the oracle's C cell list (pinned by brute
+    force in the oracle's own tests)."""
+    b = on.build(xw, cell, CUTOFF)
+    return b.pairs[:, :b.pair_total]
+
+
 def _problem(frames=4, beads=512, seed=4):
     xs, cells, hidden = workloads.cg_frames(frames=frames, beads=beads, seed=seed)
     rng = np.random.default_rng(seed + 1)
@@ -39,7 +46,7 @@ def _problem(frames=4, beads=512, seed=4):
     labels = {"energy": [], "forces": [], "virial": []}
     for x, cell in zip(xs, cells):
         xw = on.wrap(x, cell)[0]
-        pairs = on.brute_pairs(xw, cell, CUTOFF).T
+        pairs = _pairs(xw, cell)
         e, f, w = op.lj_efw(xw, cell, pairs, np.zeros(len(xw), np.int64), [hidden["sigma"]],
                             [hidden["epsilon"]], CUTOFF)
         labels["energy"].append(e)
@@ -55,7 +62,7 @@ def _oracle_predict(xs, cells, table):
     out = {"energy": [], "forces": [], "virial": []}
     for x, cell in zip(xs, cells):
         xw = on.wrap(x, cell)[0]
-        pairs = on.brute_pairs(xw, cell, CUTOFF).T
+        pairs = _pairs(xw, cell)
         e, f, w = op.table_efw(xw, cell, pairs, np.zeros(len(xw), np.int64), table[None],
                                np.array([START]), CUTOFF, spacing=spacing)
         out["energy"].append(e)
@@ -74,28 +81,30 @@ def _np_loss(pred, labels, scales):
 
 def _autograd_gradient(xs, cells, table, labels, scales, dtype=torch.float64):
     """The reference's route restated in torch (fp64 unless asked): nested autograd through
-    E(x, strain); the pair set is fixed in fp64."""
+    E(x, strain), the loss of ``training/_loss.py:105-125``, then d/dtable. The loss is a mean
+    over frames of per-frame terms, so its gradient is accumulated frame by frame."""
     t = torch.tensor(table, dtype=dtype, requires_grad=True)
     spacing = torch.tensor([(CUTOFF - START) / (KNOTS - 1)], dtype=dtype)
     start = torch.tensor([START], dtype=dtype)
-    pred = {"energy": [], "forces": [], "virial": []}
-    for x, cell in zip(xs, cells):
+    count = len(xs)
+    total = torch.zeros_like(t)
+    for f, (x, cell) in enumerate(zip(xs, cells)):
         xw = on.wrap(x, cell)[0]
-        pairs = torch.tensor(on.brute_pairs(xw, cell, CUTOFF).T).long()
+        pairs = torch.tensor(_pairs(xw, cell)).long()
         pos = torch.tensor(xw, dtype=dtype, requires_grad=True)
         strain = torch.zeros((3, 3), dtype=dtype, requires_grad=True)
         types = torch.zeros(len(xw), dtype=torch.int64)
         e = op.t_table_energy(pos, strain, torch.tensor(cell, dtype=dtype), pairs, types,
                               t[None], start, spacing, CUTOFF)
         gx, gs = torch.autograd.grad(e, (pos, strain), create_graph=True)
-        pred["energy"].append(e)
-        pred["forces"].append(-gx)
-        pred["virial"].append(-gs)
-    pred = {k: torch.stack(v) for k, v in pred.items()}
-    lab = {k: torch.tensor(v, dtype=dtype) for k, v in labels.items()}
-    value, _ = fit.loss(pred, lab, WEIGHTS, scales)
-    (grad,) = torch.autograd.grad(value, t)
-    return grad.double().numpy()
+        pred = {"energy": e, "forces": -gx, "virial": -gs}
+        term = 0.0
+        for name, value in pred.items():
+            target = torch.tensor(labels[name][f], dtype=dtype)
+            term = term + WEIGHTS[name] * (((value - target) / scales[name]) ** 2).mean()
+        (grad,) = torch.autograd.grad(term / count, t)
+        total += grad
+    return total.double().numpy()
 
 
 def _frames(xs, cells, table, dtype, device):
@@ -171,3 +180,55 @@ def test_repeated_backward_is_deterministic(device):
     assert torch.isfinite(grads[0]).all()
     # fp64 atomics in LDS sum in any order: equal to rounding, not bit for bit
     assert torch.allclose(grads[0], grads[1], rtol=1e-6, atol=1e-9 * float(grads[0].abs().max()))
+
+
+def test_full_workload_gradient(device):
+    """The force-matching workload at its stated size: 64 frames of 4,096 beads (8 nm^-3), the
+    200-knot table, fp32 on the GPU. dLoss/dknots against the fp64 autograd restatement and the
+    fp32 route (the same bar as the small frames: max(1e-4, 1.5 x the fp32 route's distance from
+    fp64)), and against fp64 central differences of the oracle's loss on three knots and one
+    random direction (exact up to rounding: the loss is quadratic in the knots)."""
+    xs, cells, table, labels = _problem(frames=64, beads=4096)
+    xs = torch.tensor(xs, dtype=torch.float32).double().numpy()
+    frames = _frames(xs, cells, table, torch.float32, device)
+    scales = fit.scales(**{k: torch.tensor(v) for k, v in labels.items()})
+    t = torch.tensor(table[None], dtype=torch.float32, device=device, requires_grad=True)
+    lab = {k: torch.tensor(v, dtype=torch.float32, device=device) for k, v in labels.items()}
+    value, _ = fit.loss(frames.predict(t), lab, WEIGHTS, scales)
+    value.backward()
+    grad = t.grad[0].double().cpu().numpy()
+    assert grad[-1] == 0.0
+    auto = _autograd_gradient(xs, cells, table, labels, scales)
+    route = _autograd_gradient(xs, cells, table, labels, scales, dtype=torch.float32)
+    tol = max(1e-4, 1.5 * _rel(route, auto))
+    print(f"64 x 4096: gradient vs fp64 autograd {_rel(grad, auto):.2e}, vs the fp32 route "
+          f"{_rel(grad, route):.2e}; the fp32 route vs fp64 {_rel(route, auto):.2e}", flush=True)
+    assert _rel(grad, auto) <= tol
+
+    pairs = [_pairs(on.wrap(x, c)[0], c) for x, c in zip(xs, cells)]
+    spacing = np.array([(CUTOFF - START) / (KNOTS - 1)])
+
+    def loss_of(tab):
+        pred = {"energy": [], "forces": [], "virial": []}
+        for x, c, p in zip(xs, cells, pairs):
+            e, f, w = op.table_efw(on.wrap(x, c)[0], c, p, np.zeros(len(x), np.int64), tab[None],
+                                   np.array([START]), CUTOFF, spacing=spacing)
+            pred["energy"].append(e)
+            pred["forces"].append(f)
+            pred["virial"].append(w)
+        return _np_loss({k: np.stack(v) for k, v in pred.items()}, labels, scales)
+
+    rng = np.random.default_rng(9)
+    direction = rng.normal(size=KNOTS)
+    direction[-1] = 0.0
+    h = 0.25
+    scale = np.abs(auto).max()
+    for k in (3, 97, 180):
+        unit = np.zeros(KNOTS)
+        unit[k] = 1.0
+        fd = (loss_of(table + h * unit) - loss_of(table - h * unit)) / (2 * h)
+        assert abs(fd - auto[k]) <= 1e-8 * scale, (k, fd, auto[k])
+        assert abs(grad[k] - fd) <= tol * scale, (k, grad[k], fd)
+    fd = (loss_of(table + h * direction) - loss_of(table - h * direction)) / (2 * h)
+    assert abs(fd - auto @ direction) <= 1e-8 * scale * np.abs(direction).sum()
+    assert abs(grad @ direction - fd) <= tol * scale * np.abs(direction).sum()

@@ -130,11 +130,11 @@ def test_full_list_is_symmetric_closure(device, dtype):
         x = rng.uniform(0, 1, (n, 3)) * cell
         nl = _allocate(x, cell, 1.0, 0.2, dtype, device, full=True)
         half = on.pair_set(nl.tensors().index.cpu().numpy(), nl.num_pairs)
-        full = nl._full
+        full = nl._device.full
         counts = full["counts"].cpu().numpy()
         slices = full["slices"].cpu().numpy()
         index = full["index"].cpu().numpy()
-        rows = (nl._buffers.sorted_ids.cpu().numpy() if not nl.all_pairs else np.arange(n))
+        rows = (nl._device.buffers["sorted_ids"].cpu().numpy() if not nl.all_pairs else np.arange(n))
         got = []
         for r in range(n):
             base = slices[r // 64] + r % 64
@@ -157,10 +157,11 @@ def test_forced_rebuild_and_torch_ops_path(device):
     nl._check_status()
     assert nl._rebuild_count == 2
     # the same update through torch.ops.nabladyn_neighbors.update
-    b, r = nl._buffers, nl._reuse
-    r.reference_cell.fill_(math.inf)
-    ops().update(xd, Ld, 1.0, 1.2, r.reference_positions, r.reference_cell, r.limits,
-                 r.parameters, r.rebuild, r.rebuild_count, *b.args())
+    dev = nl._device
+    r = dev.reuse
+    r["reference_cell"].fill_(math.inf)
+    ops().update(xd, Ld, 1.0, 1.2, r["reference_positions"], r["reference_cell"], r["limits"],
+                 r["parameters"], r["rebuild"], r["rebuild_count"], *dev.args())
     torch.cuda.synchronize()
     assert nl._rebuild_count == 3
     ref = _expected(x, L, 1.2, np.float32)

@@ -83,3 +83,54 @@ def test_oversized_and_malformed_requests_are_refused(device):
     request.kind = "bogus"
     with pytest.raises(ValueError):
         _server(device).submit(request)
+
+
+def test_full_stream_alone_equals_in_the_mix(device):
+    """The serving workload at its stated size: the 64-request Poisson stream the bench serves
+    (seed 4, 40 QPS, 256 .. 262,144 atoms, 30% 500-step NVT segments, LJ or 200-knot tables,
+    fp32). Every request's result is bitwise the same served in the mix and served alone; the
+    largest LJ and table evaluations are within 1e-5 of the reference's fp32 route on the
+    oracle's pairs."""
+    import numpy as np
+
+    from oracle import neighbors as on
+    from oracle import potentials as op
+    from _parity import check_against_route, lj_route, table_route
+
+    specs = workloads.request_stream(64, 40.0, seed=4)
+    assert max(s["atoms"] for s in specs) > 200_000
+    assert sum(s["kind"] == "md" for s in specs) >= 10
+    mix = serve.make_requests(specs, device)
+    stats = serve.Server(device).run(mix)
+    assert stats["requests"] == len(specs)
+    print(f"served 64: p50 {stats['p50_latency_ms']:.1f} ms, p99 {stats['p99_latency_ms']:.1f} ms",
+          flush=True)
+    for spec, served in zip(specs, mix):
+        alone = serve.make_requests([dict(spec, arrival=0.0)], device)
+        serve.Server(device).run(alone)
+        assert _equal(served.result, alone[0].result), (spec["kind"], spec["atoms"])
+    for kind in ("lj", "table"):
+        k = max((i for i, s in enumerate(specs) if s["kind"] == "efw" and s["potential"] == kind),
+                key=lambda i: specs[i]["atoms"])
+        request = mix[k]
+        x = request.positions.cpu().numpy()
+        cell = request.cell.cpu().numpy()
+        xw = on.wrap(x, cell)[0]
+        b = on.build(xw, cell, request.model.cutoff)
+        pairs = b.pairs[:, :b.pair_total]
+        types = np.zeros(len(x), np.int64)
+        m = request.model
+        if kind == "lj":
+            params = (m.sigma.detach().cpu().numpy(), m.epsilon.detach().cpu().numpy())
+            fp64 = op.lj_efw(xw.astype(np.float64), cell.astype(np.float64), pairs, types,
+                             *params, m.cutoff)
+            route = lj_route(xw, cell, pairs, types, *params, m.cutoff)
+        else:
+            params = (m.table.detach().cpu().numpy(), m.start.cpu().numpy())
+            spacing = m.spacing.cpu().numpy()
+            fp64 = op.table_efw(xw.astype(np.float64), cell.astype(np.float64), pairs, types,
+                                *params, m.cutoff, spacing=spacing)
+            route = table_route(xw, cell, pairs, types, *params, spacing, m.cutoff)
+        got = (float(request.result["energy"]), request.result["forces"].cpu().numpy(),
+               request.result["virial"].cpu().numpy())
+        check_against_route(f"served {kind} evaluation, {len(x)} atoms", got, route, fp64)

@@ -1,9 +1,10 @@
 """``potential.sum`` on the drop-in path: a short-range LJ (rc 0.9 nm) plus a 200-knot table
 (rc 1.2 nm) on 2,000 random atoms, evaluated by ``quantity.energy_force_virial``.
 
-- Against the fp64 oracle: the sum of the parts' closed forms, each on brute-force pairs within
-  its own cutoff. fp32: max|dF| / max|F| and max|dW| / max|W| <= 3e-5, E within 3e-5 relative
-  (the table part's own bound, as in test_gpu_forces); fp64: 3e-12.
+- fp64: against the oracle's closed forms of the parts, each on brute-force pairs within its
+  own cutoff, to 3e-12. fp32: against the reference's own fp32 route of each part (autograd of
+  its energy, restated by the oracle) on the same pairs, added: max|dF| / max|F|,
+  max|dW| / max|W| and |dE| / max(1, |E|) <= 1e-5, with the fp64 errors printed beside.
 - A list per cutoff changes the cost, never the result (``potential/_sum/_potential.py:94-95``):
   the sum on a list with an inner list at 0.9 nm equals the sum on the 1.2 nm list alone, and
   the parts evaluated alone add up to the sum: to 1e-6 of max|F| (and of W, E) in fp32, 1e-12
@@ -20,6 +21,7 @@ from nabladyn_amd import md, potential, quantity
 from nabladyn_amd.neighbors import NeighborList
 from oracle import neighbors as on
 from oracle import potentials as op
+from _parity import check_against_route, lj_route, table_route
 
 pytestmark = pytest.mark.gpu
 
@@ -59,11 +61,22 @@ def test_sum_vs_oracle(device, dtype):
     E1, F1, W1 = op.lj_efw(xw, cell, near, types, f32([0.34, 0.30, 0.28]), f32([0.9, 0.6, 0.4]), 0.9)
     E2, F2, W2 = op.table_efw(xw, cell, far, types, f32(table), f32([0.3] * 3), 1.2,
                               spacing=tab.spacing.cpu().numpy())
-    tol = 3e-5 if dtype == torch.float32 else 3e-12
     F0, W0, E0 = F1 + F2, W1 + W2, E1 + E2
-    assert np.abs(F.cpu().double().numpy() - F0).max() <= tol * np.abs(F0).max()
-    assert np.abs(W.cpu().double().numpy() - W0).max() <= tol * np.abs(W0).max()
-    assert abs(float(e) - E0) <= tol * max(1.0, abs(E0))
+    if dtype == torch.float64:
+        tol = 3e-12
+        assert np.abs(F.cpu().double().numpy() - F0).max() <= tol * np.abs(F0).max()
+        assert np.abs(W.cpu().double().numpy() - W0).max() <= tol * np.abs(W0).max()
+        assert abs(float(e) - E0) <= tol * max(1.0, abs(E0))
+        return
+    # fp32: the reference's route, each part on its own pairs, added as its Sum adds them
+    native = lambda a: np.asarray(a, dtype=np.float32)
+    r1 = lj_route(xw, cell.astype(np.float32), near, types, native([0.34, 0.30, 0.28]),
+                  native([0.9, 0.6, 0.4]), 0.9)
+    r2 = table_route(xw, cell.astype(np.float32), far, types, native(table), native([0.3] * 3),
+                     tab.spacing.cpu().numpy(), 1.2)
+    route = (float(np.float32(r1[0]) + np.float32(r2[0])), r1[1] + r2[1], r1[2] + r2[2])
+    check_against_route("sum LJ 0.9 + table 1.2, fp32", (float(e), F.cpu().numpy(), W.cpu().numpy()),
+                        route, (E0, F0, W0))
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])

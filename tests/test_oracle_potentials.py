@@ -129,3 +129,25 @@ def test_table_energy_gradient_is_knot_weight():
     e = op.table_efw(xw, cell, b.pairs, types, table, start, 1.2)[0]
     assert abs((grad * table).sum() - e) < 1e-9 * max(1.0, abs(e))  # linear, no constant term
     assert grad[0, -1] == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_route_square_root_is_correctly_rounded(dtype):
+    """The route's square root is the IEEE one the reference's CUDA platform computes: for each
+    sampled x, no neighbor of the result in the dtype is closer to sqrt(x) (checked exactly in
+    rationals: r is nearest iff (r - ulp/2)^2 <= x <= (r + ulp/2)^2). Its derivative is
+    1 / (2 r)."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(0)
+    x = torch.tensor(rng.uniform(0.01, 1.5, 3000), dtype=dtype)
+    r = op.rounded_sqrt(x)
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    for xv, rv in zip(x.numpy(), r.numpy()):
+        up = Fraction(float(np.nextafter(rv, npdt(np.inf), dtype=npdt)))
+        down = Fraction(float(np.nextafter(rv, npdt(0), dtype=npdt)))
+        here = Fraction(float(rv))
+        lo, hi = (down + here) / 2, (here + up) / 2
+        assert lo * lo <= Fraction(float(xv)) <= hi * hi
+    g = torch.func.grad(lambda v: op.rounded_sqrt(v).sum())(x)
+    assert torch.equal(g, 1.0 / (2 * r))
