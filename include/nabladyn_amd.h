@@ -135,6 +135,20 @@ int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num
                  void* forces, void* virial, void* workspace, int64_t workspace_bytes,
                  ndn_stream_t stream);
 
+/* ndn_pair_efw without the workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Its scratch (the potential's coefficients, then 64 bytes per
+ * block of the pair launch, at most 2,048 blocks) is a static device array of
+ * NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a call whose
+ * potential and list need more is refused with NDN_ERR_ARGUMENT. It calls no allocator.
+ * Nothing in the scratch needs clearing between calls (each call writes what it reads), but
+ * every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two threads) race and must be serialised by the caller. */
+#define NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES (1 << 20)
+int ndn_pair_efw_noalloc(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                         const int32_t* pairs, int64_t capacity, const ndn_potential* pot,
+                         void* energy, void* forces, void* virial, ndn_stream_t stream);
+
 /* ndn_pair_efw over a full list (every pair within the list radius in both orientations, as
  * ndn_fill_first_frame / ndn_update write it with full_* buffers): row r is atom rows[r] (rows
  * NULL: atom r), its k-th neighbor the atom id index[slices[r / 64] + r % 64 + 64 k] for

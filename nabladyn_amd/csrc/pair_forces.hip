@@ -21,6 +21,8 @@
 #include "common.hpp"
 #include "pair_forces.hpp"
 
+#include <mutex>
+#include <string>
 #include "potentials.hpp"
 
 namespace ndn {
@@ -498,6 +500,51 @@ int pair_efw_impl(const void* positions, const void* cell, std::int64_t num_atom
     return finish<T>(partials, blocks, energy, virial, 1.0, stream);
 }
 
+// ---- the entry point that allocates nothing --------------------------------------------------
+
+// The scratch of ndn_pair_efw_noalloc: the potential's coefficients from byte 0, then one
+// partial slot (8 doubles) per block of the pair launch. File-scope device memory: every GPU the
+// library runs on holds its own copy until the process ends. Host code never passes this name
+// to a kernel (on the host it is a stand-in, not a device address); it passes the address
+// hipGetSymbolAddress returns for the device, looked up once per device and kept.
+static __device__ __align__(256) unsigned char scratch[NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+void* scratch_address[kMaxDevices] = {};
+
+// The device address of `scratch` on the stream's device, looked up on the first call there.
+int scratch_for(hipStream_t stream, void** address)
+{
+    hipDevice_t device = 0;
+    hipError_t status = hipStreamGetDevice(stream, &device);
+    if (status != hipSuccess)
+    {
+        set_error(std::string("pair_efw_noalloc: hipStreamGetDevice: ") + hipGetErrorString(status));
+        return NDN_ERR_HIP;
+    }
+    NDN_REQUIRE(device >= 0 && device < kMaxDevices, "pair_efw_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (scratch_address[device] == nullptr)
+    {
+        int current = 0;
+        status = hipGetDevice(&current);
+        if (status == hipSuccess && current != device) status = hipSetDevice(device);
+        void* found = nullptr;
+        if (status == hipSuccess) status = hipGetSymbolAddress(&found, HIP_SYMBOL(scratch));
+        if (current != device) (void)hipSetDevice(current);
+        if (status != hipSuccess || found == nullptr)
+        {
+            set_error(std::string("pair_efw_noalloc: hipGetSymbolAddress: ") +
+                      hipGetErrorString(status));
+            return NDN_ERR_HIP;
+        }
+        scratch_address[device] = found;
+    }
+    *address = scratch_address[device];
+    return NDN_OK;
+}
+
 }  // namespace ndn
 
 using namespace ndn;
@@ -562,3 +609,36 @@ int ndn_pair_efw_full(int dtype, const void* positions, const void* cell, int64_
 }
 
 }  // extern "C"
+
+extern "C" int ndn_pair_efw_noalloc(int dtype, const void* positions, const void* cell,
+                                    int64_t num_atoms, const int32_t* pairs, int64_t capacity,
+                                    const ndn_potential* pot, void* energy, void* forces,
+                                    void* virial, ndn_stream_t stream)
+{
+    const int checked = check_potential(pot, num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    NDN_REQUIRE(positions && cell && pairs, "a buffer is null");
+    NDN_REQUIRE(num_atoms >= 1 && num_atoms <= INT32_MAX, "N must satisfy 1 <= N < 2**31");
+    NDN_REQUIRE(capacity >= 1 && capacity <= INT32_MAX, "pairs must be [2, C] with int32 C");
+    const int element = dtype == NDN_F32 ? 4 : 8;
+    const std::int64_t needed = coefficient_bytes(pot, element) +
+                                static_cast<std::int64_t>(half_blocks(capacity)) * 8 *
+                                    static_cast<std::int64_t>(sizeof(double));
+    if (needed > NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES)
+    {
+        set_error("pair_efw_noalloc: the potential's coefficients and partial sums need " +
+                  std::to_string(needed) + " bytes of scratch, more than its " +
+                  std::to_string(NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES) +
+                  " (use ndn_pair_efw with a workspace)");
+        return NDN_ERR_ARGUMENT;
+    }
+    void* workspace = nullptr;
+    const int found = scratch_for(stream, &workspace);
+    if (found != NDN_OK) return found;
+    if (dtype == NDN_F32)
+        return pair_efw_impl<float>(positions, cell, num_atoms, pairs, capacity, pot, energy,
+                                    forces, virial, workspace, stream);
+    return pair_efw_impl<double>(positions, cell, num_atoms, pairs, capacity, pot, energy, forces,
+                                 virial, workspace, stream);
+}

@@ -77,3 +77,16 @@ def test_neighbor_list_validation_on_host():
         nl.allocate(torch.zeros(4, 3), torch.ones(3) * 3)
     with pytest.raises(RuntimeError, match="allocate"):
         nl.tensors()
+
+
+def test_noalloc_entry_point_refuses_what_its_scratch_cannot_hold():
+    """ndn_pair_efw_noalloc's limit: a table whose padded knots outgrow the static scratch is
+    refused with NDN_ERR_ARGUMENT before anything runs (no device memory is touched: the
+    pointers below are never read)."""
+    pot = _native.Potential(kind=_native.TABLE, num_types=6, atom_types=ctypes.c_void_p(8),
+                            table=ctypes.c_void_p(8), start=ctypes.c_void_p(8),
+                            spacing=ctypes.c_void_p(8), knots=8000, cutoff=1.2)
+    with pytest.raises(_native.NativeError, match="scratch"):
+        _native.call("pair_efw_noalloc", _native.F64, ctypes.c_void_p(8), ctypes.c_void_p(8),
+                     ctypes.c_int64(1000), ctypes.c_void_p(8), ctypes.c_int64(5000),
+                     ctypes.byref(pot), None, None, None, None)
