@@ -32,8 +32,10 @@ enum {
     NDN_STATUS_OVERFLOW = 8,     /* the pairs exceed the capacity; status[1] = pairs needed */
     NDN_STATUS_FULL_OVERFLOW = 16, /* the full (both-orientation) list exceeds its capacity;
                                       status[1] is unchanged; only lists with a full list set it */
-    NDN_STATUS_HALO_OVERFLOW = 32  /* (MD list) a brick's halo exceeds halo_capacity; the
+    NDN_STATUS_HALO_OVERFLOW = 32, /* (MD list) a brick's halo exceeds halo_capacity; the
                                       largest halo is at byte 656 of scan_temp */
+    NDN_STATUS_CELL_SHRANK = 64    /* (MD list, NPT) the cell shrank below what the fixed
+                                      sub-bin grid covers (widths under half the radius) */
 };
 
 /* The device buffers of one neighbor list, in the reference op's argument order
@@ -135,20 +137,6 @@ int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num
                  void* forces, void* virial, void* workspace, int64_t workspace_bytes,
                  ndn_stream_t stream);
 
-/* ndn_pair_efw without the workspace argument, for callers that cannot allocate device memory
- * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
- * counterpart in the reference. Its scratch (the potential's coefficients, then 64 bytes per
- * block of the pair launch, at most 2,048 blocks) is a static device array of
- * NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a call whose
- * potential and list need more is refused with NDN_ERR_ARGUMENT. It calls no allocator.
- * Nothing in the scratch needs clearing between calls (each call writes what it reads), but
- * every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams,
- * two threads) race and must be serialised by the caller. */
-#define NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES (1 << 20)
-int ndn_pair_efw_noalloc(int dtype, const void* positions, const void* cell, int64_t num_atoms,
-                         const int32_t* pairs, int64_t capacity, const ndn_potential* pot,
-                         void* energy, void* forces, void* virial, ndn_stream_t stream);
-
 /* ndn_pair_efw over a full list (every pair within the list radius in both orientations, as
  * ndn_fill_first_frame / ndn_update write it with full_* buffers): row r is atom rows[r] (rows
  * NULL: atom r), its k-th neighbor the atom id index[slices[r / 64] + r % 64 + 64 k] for
@@ -186,6 +174,33 @@ int ndn_pair_parameter_gradient(int dtype, const void* positions, const void* ce
                                 int64_t out_count, void* workspace, int64_t workspace_bytes,
                                 ndn_stream_t stream);
 
+/* The vector-Jacobian product of (E, F, W) = ndn_pair_efw(...) with respect to the parameters,
+ * in fp64: out = gE dE/dtheta + sum gF . dF/dtheta + sum gW : dW/dtheta, given the upstream
+ * grad_energy [1], grad_forces [N, 3] and grad_virial [3, 3] (T, device; any may be NULL for
+ * zero). Layout of out as for ndn_pair_parameter_gradient. It is the exact mixed second
+ * derivative of force and virial matching, per pair (F_i = sum_j fs d_ij, W = sum fs d (x) d):
+ * gE du/dtheta + ((gF_i - gF_j) . d + d^T gW d) dfs/dtheta. Replaces the nested
+ * torch.func.grad of the reference's fit (training/_loss.py:71-125, training/_fit.py:245-271)
+ * for LJ and tabulated models. The workspace is ndn_pair_workspace_bytes. */
+int ndn_pair_parameter_vjp(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                           const int32_t* pairs, int64_t capacity, const ndn_potential* pot,
+                           const void* grad_energy, const void* grad_forces,
+                           const void* grad_virial, double* out, int64_t out_count,
+                           void* workspace, int64_t workspace_bytes, ndn_stream_t stream);
+
+/* ndn_pair_efw without the workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Its scratch (the potential's coefficients, then 64 bytes per
+ * block of the pair launch, at most 2,048 blocks) is a static device array of
+ * NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a call whose
+ * potential and list need more is refused with NDN_ERR_ARGUMENT. It calls no allocator.
+ * Nothing in the scratch needs clearing between calls (each call writes what it reads), but
+ * every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two threads) race and must be serialised by the caller. */
+#define NDN_PAIR_EFW_NOALLOC_SCRATCH_BYTES (1 << 20)
+int ndn_pair_efw_noalloc(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                         const int32_t* pairs, int64_t capacity, const ndn_potential* pot,
+                         void* energy, void* forces, void* virial, ndn_stream_t stream);
 
 /* ---- molecular dynamics ----------------------------------------------------------------- */
 
@@ -213,6 +228,9 @@ typedef struct ndn_md_system {
     int64_t partial_slots;
     void* coefficients;
     int64_t coefficient_bytes;
+    double* barostat;           /* NPT: [16] (box momentum, its chain's q [3], p [3] and masses
+                                   [3], the master cell [3], this step's drag and growth);
+                                   NULL otherwise. The cell is then rewritten every step. */
 } ndn_md_system;
 
 typedef struct ndn_md_settings {
@@ -220,7 +238,11 @@ typedef struct ndn_md_settings {
     double kT;                    /* kJ/mol (NVT) */
     int64_t degrees_of_freedom;   /* 3N - 3 */
     int thermostat;               /* 0: NVE, 1: NVT with a Nose-Hoover chain of 3 */
-    int virial;                   /* also compute W each step */
+    int virial;                   /* also compute W each step (NPT: required) */
+    /* NPT, isotropic MTK (md/_ensembles/_npt.py:66-266), used when system.barostat is set */
+    double pressure;              /* kJ/mol/nm^3 */
+    double box_mass;              /* 3 (N + 1) kT tau_b^2 */
+    double alpha;                 /* 1 + 1/N */
 } ndn_md_settings;
 
 /* The reuse state of a neighbor list (neighbors/_neighbor_list.py:42-56). */
@@ -285,6 +307,8 @@ typedef struct ndn_md_list {
     int32_t* halo_slots;        /* [bricks * halo_capacity] */
     int32_t* halo_runs;         /* [bricks * 128] local start of each run of the halo */
     int32_t* halo_counts;       /* [bricks] */
+    void* grid_widths;          /* [3] T device: the sub-bin widths cell / dims (NPT: the
+                                   prologue rewrites them each step) */
 } ndn_md_list;
 
 /* Scratch sizes of a system: partial slots and potential-coefficient bytes. */

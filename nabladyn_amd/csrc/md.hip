@@ -53,6 +53,17 @@ constexpr int kPending = 6;
 constexpr int kOpening = 7;
 constexpr int kStencil = 5;          // sub-bins per axis a stencil spans
 
+// Barostat slots (fp64, NPT only): the box momentum p_eps [0], its chain's positions [1..3] and
+// momenta [4..6], that chain's masses [7..9], the master cell [10..12], and this step's drag
+// exp(-alpha v_eps dt/2) [13] and growth exp(v_eps dt/2) [14].
+constexpr int kBoxP = 0;
+constexpr int kBoxChainQ = 1;
+constexpr int kBoxChainP = 4;
+constexpr int kBoxChainMass = 7;
+constexpr int kMasterCell = 10;
+constexpr int kDrag = 13;
+constexpr int kGrowth = 14;
+
 // The chain's half step against the kinetic energy K it thermostats (_chain.py:29-76), in the
 // reference's operation order. Updates q, p and K; returns the velocity factor.
 __device__ __forceinline__ double chain_half_step(double* kinetic, double* q, double* p,
@@ -378,6 +389,12 @@ struct StepArgs
     double degrees;
     double timestep;
     int nvt;
+    // barostat (NPT: barostat non-null; the cell then changes every step)
+    double* barostat;
+    double pressure;
+    double box_mass;
+    double alpha;
+    T* cell_out;
     // reuse rule
     T cutoff;
     T radius;
@@ -394,7 +411,7 @@ struct StepArgs
     index_t* total_scratch;
     ListHeader* header;
     index_t nx, ny, nz;
-    T wx, wy, wz;
+    T* grid_widths;  // [3] sub-bin widths, the cell over nx, ny, nz (NPT: rewritten each step)
     std::int64_t bins;
     std::int64_t stride;
     index_t* bin_counts;
@@ -466,6 +483,65 @@ __device__ __forceinline__ Launch own_launch()
     return Launch{static_cast<index_t>(blockIdx.x), static_cast<index_t>(gridDim.x)};
 }
 
+// ---- the barostat (NPT) ---------------------------------------------------------------------
+
+// G = alpha 2K + trace W - 3 P V (md/_ensembles/_npt.py:232-238): the carried K, the trace of
+// the frame's virial in its dtype, the master cell's volume.
+template <typename T>
+__device__ __forceinline__ double box_force(const StepArgs<T>& a)
+{
+#pragma clang fp contract(off)
+    const double* b = a.barostat;
+    const T trace = (a.virial[0] + a.virial[4]) + a.virial[8];
+    const double volume = (b[kMasterCell] * b[kMasterCell + 1]) * b[kMasterCell + 2];
+    return ((a.alpha * 2.0) * a.kinetic[0] + static_cast<double>(trace)) -
+           (3.0 * a.pressure) * volume;
+}
+
+// The barostat chain's half step against the box's kinetic energy p_eps^2 / (2M); scales p_eps
+// (_npt.py:257-268).
+template <typename T>
+__device__ __forceinline__ void box_chain_half_step(const StepArgs<T>& a)
+{
+#pragma clang fp contract(off)
+    double* b = a.barostat;
+    double kinetic = b[kBoxP] * b[kBoxP] / (2.0 * a.box_mass);
+    const double factor = chain_half_step(&kinetic, b + kBoxChainQ, b + kBoxChainP,
+                                          b + kBoxChainMass, 1.0, a.kT, a.timestep);
+    b[kBoxP] = b[kBoxP] * factor;
+}
+
+// The opening half of an NPT step on the box (_npt.py:154-174, after the particles' chain): the
+// box momentum's kick, this step's drag and growth, the cell grown twice by the growth, and the
+// sub-bin widths of the new cell. A width below what the fixed stencil covers sets
+// NDN_STATUS_CELL_SHRANK.
+template <typename T>
+__device__ __forceinline__ void box_first_half(const StepArgs<T>& a)
+{
+#pragma clang fp contract(off)
+    double* b = a.barostat;
+    const double half = 0.5 * a.timestep;
+    b[kBoxP] = b[kBoxP] + half * box_force(a);
+    const double rate = b[kBoxP] / a.box_mass;
+    b[kDrag] = exp(-a.alpha * rate * half);
+    const double growth = exp(rate * half);
+    b[kGrowth] = growth;
+    const index_t dims[3] = {a.nx, a.ny, a.nz};
+    const double epsilon = Epsilon<T>::value;
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        b[kMasterCell + axis] = (b[kMasterCell + axis] * growth) * growth;
+        const T length = static_cast<T>(b[kMasterCell + axis]);
+        a.cell_out[axis] = length;
+        const T width = length / static_cast<T>(dims[axis]);
+        a.grid_widths[axis] = width;
+        const double needed = 0.5 * static_cast<double>(a.radius) * (1.0 + 1e-6) +
+                              4.0 * epsilon * static_cast<double>(length);
+        if (static_cast<double>(width) < needed && dims[axis] > 1)
+            atomicOr(a.status, NDN_STATUS_CELL_SHRANK);
+    }
+}
+
 // ---- prologue -------------------------------------------------------------------------------
 
 template <typename T>
@@ -474,12 +550,15 @@ __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_po
 {
     if (threadIdx.x == 0)
     {
+        // NPT: the barostat's chain, the particles' chain, then the box (_npt.py:154-174)
+        if (a.barostat) box_chain_half_step(a);
         if (a.nvt)
         {
             double* t = a.thermostat;
             t[kOpening] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
                                           a.degrees, a.kT, a.timestep);
         }
+        if (a.barostat) box_first_half(a);
         prepare_update(a.cell, a.reference_cell, a.cutoff, a.radius, a.limits, a.dims_scratch,
                        a.widths_scratch, a.reuse, a.rebuild, a.status, a.total_scratch, a.header);
         if (a.inner_index)
@@ -577,6 +656,16 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
             vy = vy * opening;
             vz = vz * opening;
         }
+        T scale = T(1);
+        if (a.barostat)
+        {
+            // NPT: the drag before the kick, and the box's growth on both sides of the drift
+            const T drag = static_cast<T>(a.barostat[kDrag]);
+            vx = vx * drag;
+            vy = vy * drag;
+            vz = vz * drag;
+            scale = static_cast<T>(a.barostat[kGrowth]);
+        }
         vx = vx + half_timestep * f[0] * inverse_mass;
         vy = vy + half_timestep * f[1] * inverse_mass;
         vz = vz + half_timestep * f[2] * inverse_mass;
@@ -587,7 +676,8 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
         for (int axis = 0; axis < 3; ++axis)
         {
             const T length = a.cell[axis];
-            const T drifted = x[axis] + timestep * vel[axis];
+            const T drifted = a.barostat ? (x[axis] * scale + timestep * vel[axis]) * scale
+                                         : x[axis] + timestep * vel[axis];
             T k = floor(drifted / length);
             T w = drifted - k * length;
             if (w < T(0))
@@ -710,9 +800,10 @@ __device__ __forceinline__ void md_assign_body(const StepArgs<T>& a, const Launc
          slot < a.num_atoms; slot += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
     {
         const T* p = a.positions + 3 * slot;
+        const T* w = a.grid_widths;
         const index_t bin = static_cast<index_t>(
-            sub_bin(p[0], a.wx, a.nx) + row_first_bin(sub_bin(p[1], a.wy, a.ny),
-                                                      sub_bin(p[2], a.wz, a.nz), a.nx, a.ny, a.nz));
+            sub_bin(p[0], w[0], a.nx) + row_first_bin(sub_bin(p[1], w[1], a.ny),
+                                                      sub_bin(p[2], w[2], a.nz), a.nx, a.ny, a.nz));
         a.bin_of[slot] = bin;
         atomicAdd(a.bin_counts + bin, 1);
     }
@@ -1232,7 +1323,7 @@ __device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const
 {
     BrickFrame<T> f;
     f.exact = true;
-    const T widths[3] = {a.wx, a.wy, a.wz};
+    const T widths[3] = {a.grid_widths[0], a.grid_widths[1], a.grid_widths[2]};
     for (int axis = 0; axis < 3; ++axis)
     {
         const T w = widths[axis];
@@ -1307,15 +1398,15 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
         if (row < span_y * span_z)
         {
             const int oy = row % span_y, oz = row / span_y;
-            const T dy = full_y ? T(0) : bin_gap(oy - 2, a.wy);
-            const T dz = full_z ? T(0) : bin_gap(oz - 2, a.wz);
+            const T dy = full_y ? T(0) : bin_gap(oy - 2, a.grid_widths[1]);
+            const T dz = full_z ? T(0) : bin_gap(oz - 2, a.grid_widths[2]);
             const T dyz = dy * dy + dz * dz;
             if (dyz < cull_squared)
             {
                 int first = span_x, last = -1;
                 for (int ox = 0; ox < span_x; ++ox)
                 {
-                    const T dx = full_x ? T(0) : bin_gap(ox - 2, a.wx);
+                    const T dx = full_x ? T(0) : bin_gap(ox - 2, a.grid_widths[0]);
                     if (dx * dx + dyz < cull_squared)
                     {
                         first = ox < first ? ox : first;
@@ -1950,9 +2041,17 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
 #pragma clang fp contract(off)
                 T* v = a.velocities + 3LL * slot;
                 const T inverse_mass = a.inverse_masses[slot];
-                const T vx = v[0] + half_timestep * fx * inverse_mass;
-                const T vy = v[1] + half_timestep * fy * inverse_mass;
-                const T vz = v[2] + half_timestep * fz * inverse_mass;
+                T vx = v[0] + half_timestep * fx * inverse_mass;
+                T vy = v[1] + half_timestep * fy * inverse_mass;
+                T vz = v[2] + half_timestep * fz * inverse_mass;
+                if (a.barostat)
+                {
+                    // NPT: the drag after the kick (_npt.py:176-186)
+                    const T drag = static_cast<T>(a.barostat[kDrag]);
+                    vx = vx * drag;
+                    vy = vy * drag;
+                    vz = vz * drag;
+                }
                 v[0] = vx;
                 v[1] = vy;
                 v[2] = vz;
@@ -2082,12 +2181,20 @@ __device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
         for (int k = 0; k < 9; ++k) a.virial[k] = static_cast<T>(w[k]);
     }
     if (kinetic_from_partials && a.kinetic) a.kinetic[0] = sums[7];
+    // NPT closing (_npt.py:188-191): the box momentum's kick with the new K and W, then the
+    // particles' chain, then the barostat's
+    if (a.barostat && kinetic_from_partials)
+    {
+#pragma clang fp contract(off)
+        a.barostat[kBoxP] = a.barostat[kBoxP] + (0.5 * a.timestep) * box_force(a);
+    }
     if (a.nvt && kinetic_from_partials)
     {
         double* t = a.thermostat;
         t[kPending] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
                                       a.degrees, a.kT, a.timestep);
     }
+    if (a.barostat && kinetic_from_partials) box_chain_half_step(a);
 }
 
 template <typename T>
@@ -2152,6 +2259,14 @@ StepArgs<T> step_args(const ndn_md_system& s, const ndn_md_settings* settings,
         a.degrees = static_cast<double>(settings->degrees_of_freedom);
         a.timestep = settings->timestep;
         a.nvt = settings->thermostat;
+        if (s.barostat != nullptr)
+        {
+            a.barostat = s.barostat;
+            a.pressure = settings->pressure;
+            a.box_mass = settings->box_mass;
+            a.alpha = settings->alpha;
+            a.cell_out = static_cast<T*>(const_cast<void*>(s.cell));
+        }
     }
     if (list)
     {
@@ -2172,9 +2287,7 @@ StepArgs<T> step_args(const ndn_md_system& s, const ndn_md_settings* settings,
     a.nx = m.dims[0];
     a.ny = m.dims[1];
     a.nz = m.dims[2];
-    a.wx = static_cast<T>(m.widths[0]);
-    a.wy = static_cast<T>(m.widths[1]);
-    a.wz = static_cast<T>(m.widths[2]);
+    a.grid_widths = static_cast<T*>(m.grid_widths);
     a.bins = static_cast<std::int64_t>(m.dims[0]) * m.dims[1] * m.dims[2];
     a.stride = m.stride;
     a.bin_counts = m.bin_counts;
@@ -2490,7 +2603,8 @@ int check_system(const ndn_md_system* s, const ndn_md_list* m)
                 "a system buffer is null");
     NDN_REQUIRE(m->bin_counts && m->bin_offsets && m->bin_of && m->unordered && m->order &&
                     m->counts && m->ids && m->positions4 && m->scratch && m->scan_temp &&
-                    m->status && m->prepare_dims && m->prepare_widths && m->prepare_total,
+                    m->status && m->prepare_dims && m->prepare_widths && m->prepare_total &&
+                    m->grid_widths,
                 "a list buffer is null");
     NDN_REQUIRE(m->dims[0] >= 1 && m->dims[1] >= 1 && m->dims[2] >= 1, "bad sub-bin dims");
     const std::int64_t bins = static_cast<std::int64_t>(m->dims[0]) * m->dims[1] * m->dims[2];
@@ -3129,6 +3243,15 @@ int ndn_md_step(int dtype, const ndn_md_system* system, const ndn_md_settings* s
     NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
     NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
                 "several atom types need the slot-ordered types");
+    if (system->barostat != nullptr)
+    {
+        NDN_REQUIRE(settings->thermostat && system->chain_masses,
+                    "NPT thermostats the particles: it needs the chain and its masses");
+        NDN_REQUIRE(settings->virial && system->virial, "NPT needs the virial every step");
+        NDN_REQUIRE(md_list->inner_index == nullptr, "NPT keeps no pruned list");
+        NDN_REQUIRE(settings->box_mass > 0 && settings->alpha > 0,
+                    "NPT needs a positive box mass and alpha");
+    }
     if (dtype == NDN_F32)
         return md_step_impl<float>(*system, settings, pot, list, *md_list, stream);
     NDN_REQUIRE(dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
@@ -3229,6 +3352,7 @@ int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int3
         const ndn_md_member& m = members[i];
         NDN_REQUIRE(m.system && m.settings && m.potential && m.list && m.md_list,
                     "a member's structure is null");
+        NDN_REQUIRE(m.system->barostat == nullptr, "a batch steps NVE and NVT members only");
         const int checked = check_step(dtype, m.system, m.settings, m.potential, m.list, m.md_list);
         if (checked != NDN_OK) return checked;
         NDN_REQUIRE(m.md_list->group == members[0].md_list->group,

@@ -21,6 +21,14 @@
 //        derivative is 0; potential/_tabulated/_potential.py:59-91).
 // Sums are in fp64: per block in LDS when the rows fit, else straight to memory, with atomics
 // (so the last bits depend on the order blocks finish).
+//
+// The vector-Jacobian product for fitting (training/_loss.py:94-125, whose loss reads E, F and
+// W): with upstream gE, gF [N, 3] and gW [3, 3], a pair of vector d = x_i - x_j contributes
+//   gE du/dtheta + ((gF_i - gF_j) . d + d^T gW d) dfs/dtheta,
+// since F_i = sum_j fs d_ij and W = sum fs d (x) d. dfs/dtheta:
+//   LJ   fs = 24 eps (2 p^2 - p) / r^2:  dfs/deps = 24 (2 p^2 - p) / r^2,
+//        dfs/dsigma = 24 eps (24 p^2 - 6 p) / (sigma r^2)
+//   table fs = -(du/dt) / (h r), du/dt linear in the knots with the Catmull-Rom slope weights.
 
 #include <algorithm>
 #include <climits>
@@ -58,6 +66,39 @@ __device__ __forceinline__ void strain_vector(const Strain<T>& s, const T d[3], 
         out[a] = second + d[2] * s.e[3 * a + 2];
     }
 }
+
+// The upstream gradients of a vector-Jacobian product; all null: the plain dE/dtheta.
+template <typename T>
+struct Upstream
+{
+    const T* energy;   // [1] or null (0)
+    const T* forces;   // [N, 3] or null
+    const T* virial;   // [3, 3] or null
+    bool plain;        // dE/dtheta: gE = 1, no F or W term
+
+    // gE and the slope weight (gF_i - gF_j) . d + d^T gW d of a pair
+    __device__ __forceinline__ void weights(index_t i, index_t j, const T d[3], double& ge,
+                                            double& slope) const
+    {
+        if (plain)
+        {
+            ge = 1.0;
+            slope = 0.0;
+            return;
+        }
+        ge = energy != nullptr ? static_cast<double>(energy[0]) : 0.0;
+        slope = 0.0;
+        if (forces != nullptr)
+            for (int a = 0; a < 3; ++a)
+                slope += static_cast<double>(forces[3LL * i + a] - forces[3LL * j + a]) *
+                         static_cast<double>(d[a]);
+        if (virial != nullptr)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    slope += static_cast<double>(d[a]) * static_cast<double>(virial[3 * a + b]) *
+                             static_cast<double>(d[b]);
+    }
+};
 
 // The pair's minimum-image vector d and, strained, d' and r^2 = |d'|^2 (exact arithmetic).
 template <typename T, bool kStrained>
@@ -176,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) lj_parameter_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell, const index_t* __restrict__ pairs,
     std::int64_t capacity, index_t num_atoms, T cutoff_squared, const T* __restrict__ sigma,
     const T* __restrict__ epsilon, const std::int64_t* __restrict__ types, int num_types,
-    const T* __restrict__ strain_matrix, double* __restrict__ out)
+    const T* __restrict__ strain_matrix, Upstream<T> up, double* __restrict__ out)
 {
     __shared__ double local_store[kLocalGradient];
     const Strain<T> strain = read_strain(strain_matrix);
@@ -204,8 +245,18 @@ __global__ void __launch_bounds__(kBlock) lj_parameter_kernel(
         const double c = (s2 / static_cast<double>(cutoff_squared)) *
                          (s2 / static_cast<double>(cutoff_squared)) *
                          (s2 / static_cast<double>(cutoff_squared));
-        add_gradient(local, out, row, 24.0 * e / s * ((2.0 * p * p - p) - (2.0 * c * c - c)));
-        add_gradient(local, out, rows + row, 4.0 * (p * p - p) - 4.0 * (c * c - c));
+        double ge, slope;
+        up.weights(i, j, dp, ge, slope);
+        const double inverse_r2 = 1.0 / static_cast<double>(r2);
+        double d_sigma = ge * (24.0 * e / s * ((2.0 * p * p - p) - (2.0 * c * c - c)));
+        double d_epsilon = ge * (4.0 * (p * p - p) - 4.0 * (c * c - c));
+        if (slope != 0.0)
+        {
+            d_sigma += slope * (24.0 * e * (24.0 * p * p - 6.0 * p) * inverse_r2 / s);
+            d_epsilon += slope * (24.0 * (2.0 * p * p - p) * inverse_r2);
+        }
+        add_gradient(local, out, row, d_sigma);
+        add_gradient(local, out, rows + row, d_epsilon);
     }
     if (local != nullptr)
     {
@@ -220,7 +271,7 @@ template <typename T, bool kStrained>
 __global__ void __launch_bounds__(kBlock) table_parameter_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell, const index_t* __restrict__ pairs,
     std::int64_t capacity, index_t num_atoms, T cutoff_squared, Table<T> table,
-    const T* __restrict__ strain_matrix, double* __restrict__ out)
+    const T* __restrict__ strain_matrix, Upstream<T> up, double* __restrict__ out)
 {
     __shared__ double local_store[kLocalGradient];
     const Strain<T> strain = read_strain(strain_matrix);
@@ -249,22 +300,39 @@ __global__ void __launch_bounds__(kBlock) table_parameter_kernel(
         segment = segment < T(0) ? T(0) : (segment > T(knots - 2) ? T(knots - 2) : segment);
         const double t = static_cast<double>(x - segment);
         const double s = t > 0.0 ? t : 0.0;
-        double weight[4];
+        double ge, slope_weight;
+        up.weights(i, j, dp, ge, slope_weight);
+        // du/du_m (value weights) and d(du/dt)/du_m (slope weights) of the padded knots
+        double value[4], slope[4];
         if (t < 0.0)
         {
-            weight[0] = -0.5 * t;
-            weight[1] = 1.0;
-            weight[2] = 0.5 * t;
-            weight[3] = 0.0;
+            value[0] = -0.5 * t;
+            value[1] = 1.0;
+            value[2] = 0.5 * t;
+            value[3] = 0.0;
+            slope[0] = -0.5;
+            slope[1] = 0.0;
+            slope[2] = 0.5;
+            slope[3] = 0.0;
         }
         else
         {
             const double s2 = s * s, s3 = s2 * s;
-            weight[0] = -0.5 * s + s2 - 0.5 * s3;
-            weight[1] = 1.0 - 2.5 * s2 + 1.5 * s3;
-            weight[2] = 0.5 * s + 2.0 * s2 - 1.5 * s3;
-            weight[3] = -0.5 * s2 + 0.5 * s3;
+            value[0] = -0.5 * s + s2 - 0.5 * s3;
+            value[1] = 1.0 - 2.5 * s2 + 1.5 * s3;
+            value[2] = 0.5 * s + 2.0 * s2 - 1.5 * s3;
+            value[3] = -0.5 * s2 + 0.5 * s3;
+            slope[0] = -0.5 + 2.0 * s - 1.5 * s2;
+            slope[1] = -5.0 * s + 4.5 * s2;
+            slope[2] = 0.5 + 4.0 * s - 4.5 * s2;
+            slope[3] = -s + 1.5 * s2;
         }
+        // fs = -(du/dt) / (h r)
+        const double fs_scale = slope_weight != 0.0
+            ? -slope_weight / (static_cast<double>(table.spacing[row]) * static_cast<double>(r))
+            : 0.0;
+        double weight[4];
+        for (int m = 0; m < 4; ++m) weight[m] = ge * value[m] + fs_scale * slope[m];
         const std::int64_t base = static_cast<std::int64_t>(row) * knots;
         const int first = static_cast<int>(segment);  // padded knot of weight[0]
         for (int m = 0; m < 4; ++m)
@@ -344,8 +412,8 @@ int strained_impl(const void* positions, const void* cell, std::int64_t num_atom
 template <typename T>
 int parameter_impl(const void* positions, const void* cell, std::int64_t num_atoms,
                    const index_t* pairs, std::int64_t capacity, const ndn_potential* pot,
-                   const void* strain, double* out, std::int64_t out_count, void* workspace,
-                   hipStream_t stream)
+                   const void* strain, const Upstream<T>& up, double* out,
+                   std::int64_t out_count, void* workspace, hipStream_t stream)
 {
     PotentialView<T> view;
     const int prepared = prepare_potential<T>(pot, workspace, stream, &view);
@@ -366,18 +434,18 @@ int parameter_impl(const void* positions, const void* cell, std::int64_t num_ato
         if (strained)
             lj_parameter_kernel<T, true><<<blocks, kBlock, 0, stream>>>(
                 x, L, pairs, capacity, n, view.cutoff_squared, sigma, epsilon, pot->atom_types,
-                pot->num_types, e, out);
+                pot->num_types, e, up, out);
         else
             lj_parameter_kernel<T, false><<<blocks, kBlock, 0, stream>>>(
                 x, L, pairs, capacity, n, view.cutoff_squared, sigma, epsilon, pot->atom_types,
-                pot->num_types, e, out);
+                pot->num_types, e, up, out);
     }
     else if (strained)
         table_parameter_kernel<T, true><<<blocks, kBlock, 0, stream>>>(
-            x, L, pairs, capacity, n, view.cutoff_squared, view.table, e, out);
+            x, L, pairs, capacity, n, view.cutoff_squared, view.table, e, up, out);
     else
         table_parameter_kernel<T, false><<<blocks, kBlock, 0, stream>>>(
-            x, L, pairs, capacity, n, view.cutoff_squared, view.table, e, out);
+            x, L, pairs, capacity, n, view.cutoff_squared, view.table, e, up, out);
     NDN_LAUNCHED("parameter_gradient");
     return NDN_OK;
 }
@@ -436,9 +504,39 @@ int ndn_pair_parameter_gradient(int dtype, const void* positions, const void* ce
                 "out must hold 2P doubles for LJ (dsigma, depsilon) or P * K for a table");
     if (dtype == NDN_F32)
         return parameter_impl<float>(positions, cell, num_atoms, pairs, capacity, pot, strain,
-                                     out, out_count, workspace, stream);
-    return parameter_impl<double>(positions, cell, num_atoms, pairs, capacity, pot, strain, out,
+                                     Upstream<float>{nullptr, nullptr, nullptr, true}, out,
+                                     out_count, workspace, stream);
+    return parameter_impl<double>(positions, cell, num_atoms, pairs, capacity, pot, strain,
+                                  Upstream<double>{nullptr, nullptr, nullptr, true}, out,
                                   out_count, workspace, stream);
+}
+
+int ndn_pair_parameter_vjp(int dtype, const void* positions, const void* cell, int64_t num_atoms,
+                           const int32_t* pairs, int64_t capacity, const ndn_potential* pot,
+                           const void* grad_energy, const void* grad_forces,
+                           const void* grad_virial, double* out, int64_t out_count,
+                           void* workspace, int64_t workspace_bytes, ndn_stream_t stream)
+{
+    const int checked = check_pair_call(dtype, pot, positions, cell, num_atoms, pairs, capacity,
+                                        workspace, workspace_bytes);
+    if (checked != NDN_OK) return checked;
+    const std::int64_t rows = static_cast<std::int64_t>(pot->num_types) * (pot->num_types + 1) / 2;
+    const std::int64_t expected = pot->kind == NDN_LJ ? 2 * rows : rows * pot->knots;
+    NDN_REQUIRE(out != nullptr && out_count == expected,
+                "out must hold 2P doubles for LJ (dsigma, depsilon) or P * K for a table");
+    if (dtype == NDN_F32)
+        return parameter_impl<float>(
+            positions, cell, num_atoms, pairs, capacity, pot, nullptr,
+            Upstream<float>{static_cast<const float*>(grad_energy),
+                            static_cast<const float*>(grad_forces),
+                            static_cast<const float*>(grad_virial), false},
+            out, out_count, workspace, stream);
+    return parameter_impl<double>(
+        positions, cell, num_atoms, pairs, capacity, pot, nullptr,
+        Upstream<double>{static_cast<const double*>(grad_energy),
+                         static_cast<const double*>(grad_forces),
+                         static_cast<const double*>(grad_virial), false},
+        out, out_count, workspace, stream);
 }
 
 }  // extern "C"

@@ -18,7 +18,7 @@ from torch import Tensor
 from .. import _native, space
 from ..potential import Potential
 from ..units import BOLTZMANN
-from ._state import ChainState, State
+from ._state import BarostatState, ChainState, State
 
 _STATUS_INTERVAL = 1000
 # Steps captured in the multi-step graph: one replay launches this many steps, so the host
@@ -47,7 +47,7 @@ class MDSystem(_native.ctypes.Structure):
         ("thermostat", _native.ctypes.c_void_p), ("chain_masses", _native.ctypes.c_void_p),
         ("kinetic", _native.ctypes.c_void_p), ("partials", _native.ctypes.c_void_p),
         ("partial_slots", _native.ctypes.c_int64), ("coefficients", _native.ctypes.c_void_p),
-        ("coefficient_bytes", _native.ctypes.c_int64),
+        ("coefficient_bytes", _native.ctypes.c_int64), ("barostat", _native.ctypes.c_void_p),
     ]
 
 
@@ -57,7 +57,8 @@ class MDSettings(_native.ctypes.Structure):
     _fields_ = [
         ("timestep", _native.ctypes.c_double), ("kT", _native.ctypes.c_double),
         ("degrees_of_freedom", _native.ctypes.c_int64), ("thermostat", _native.ctypes.c_int),
-        ("virial", _native.ctypes.c_int),
+        ("virial", _native.ctypes.c_int), ("pressure", _native.ctypes.c_double),
+        ("box_mass", _native.ctypes.c_double), ("alpha", _native.ctypes.c_double),
     ]
 
 
@@ -92,11 +93,18 @@ class MDList(_native.ctypes.Structure):
         ("brick", _native.ctypes.c_int32 * 3), ("bricks", _native.ctypes.c_int64),
         ("halo_capacity", _native.ctypes.c_int64), ("halo_slots", _native.ctypes.c_void_p),
         ("halo_runs", _native.ctypes.c_void_p), ("halo_counts", _native.ctypes.c_void_p),
+        ("grid_widths", _native.ctypes.c_void_p),
     ]
 
 
 _GROUPS = (4, 8, 16)
 _INVALID_CELL, _NON_FINITE, _OVERFLOW, _FULL_OVERFLOW, _HALO_OVERFLOW = 1, 2, 8, 16, 32
+_CELL_SHRANK = 64
+# NPT: the sub-bins are chosen wider than half the list's radius by this factor, so the cell may
+# shrink by as much (about 15% in volume) before the fixed grid stops covering the radius
+NPT_GRID_MARGIN = 0.05
+# barostat slots (fp64): box momentum, its chain's q [3], p [3] and masses [3], the master cell
+_BOX_P, _BOX_Q, _BOX_PC, _BOX_MASS, _MASTER_CELL = 0, 1, 4, 7, 10
 _HALO_RUNS = 128  # halo run slots per brick
 # brick shapes (sub-bins along x, y, z), largest first: the first whose expected halo fits wins
 _BRICKS = ((4, 4, 8), (4, 2, 8), (2, 2, 8), (2, 2, 4), (2, 1, 4), (1, 1, 4), (1, 1, 2),
@@ -135,11 +143,13 @@ def halo_limit(dtype: torch.dtype) -> int:
     return int(query(_native.dtype_tag(dtype)))
 
 
-def sub_bin_dims(lengths: list[float], radius: float, dtype: torch.dtype) -> list[int]:
+def sub_bin_dims(lengths: list[float], radius: float, dtype: torch.dtype,
+                 margin: float = 0.0) -> list[int]:
     """Sub-bins per axis, each at least radius / 2 wide with a margin for the rounding of binning
-    (so every pair within the radius lies within two sub-bins along each axis)."""
+    (so every pair within the radius lies within two sub-bins along each axis); ``margin``
+    widens them further by that fraction (a cell that may shrink)."""
     eps = torch.finfo(dtype).eps
-    half = 0.5 * radius * (1.0 + 1e-6)
+    half = 0.5 * radius * (1.0 + 1e-6) * (1.0 + margin)
     return [max(1, math.floor(length / (half + 4 * eps * length))) for length in lengths]
 
 
@@ -147,7 +157,7 @@ class _MDNeighbors:
     """The MD path's neighbor state: the reuse rule's tensors and the full list in slot order."""
 
     def __init__(self, cutoff, skin, multiplier, positions, cell, types, group_hint=None,
-                 inner_skin=None, reserve=0):
+                 inner_skin=None, reserve=0, grid_margin=0.0, halo_margin=None):
         from ..neighbors._neighbor_list import grid_dims
 
         # LDS kept for the potential beside the halo, in halo positions
@@ -160,7 +170,10 @@ class _MDNeighbors:
         lengths = [float(v) for v in cell.tolist()]
         ints = {"dtype": torch.int32, "device": device}
         self.limits = torch.tensor(grid_dims(lengths, self.radius, dtype), **ints)
-        dims = sub_bin_dims(lengths, self.radius, dtype)
+        dims = sub_bin_dims(lengths, self.radius, dtype, grid_margin)
+        # headroom of the halos over the first build's largest (half the rows' by default)
+        self.halo_margin = (1.0 + 0.5 * (self.multiplier - 1.0) if halo_margin is None
+                            else float(halo_margin))
         bins = math.prod(dims)
         if bins >= 2**31 - 1:
             raise ValueError("the grid must have fewer than 2**31 - 1 bins")
@@ -192,6 +205,9 @@ class _MDNeighbors:
             "prepare_dims": torch.zeros(3, **ints),
             "prepare_widths": torch.zeros(3, dtype=dtype, device=device),
             "prepare_total": torch.zeros(1, **ints),
+            # the sub-bin widths (cell / dims) the kernels read; an NPT step rewrites them
+            "grid_widths": torch.tensor([L / d for L, d in zip(lengths, dims)], dtype=dtype,
+                                        device=device),
         }
         if self.inner_skin is not None:
             b_ = self.buffers
@@ -247,7 +263,7 @@ class _MDNeighbors:
     def size_halo(self, needed: int) -> None:
         """Room for halos of `needed` atoms with a margin (half the rows' multiplier's), up to
         what LDS holds."""
-        margin = 1.0 + 0.5 * (self.multiplier - 1.0)
+        margin = self.halo_margin
         limit = halo_limit(self.dtype)
         # within the budget less the potential's share when the halo fits it (else as LDS allows)
         room = limit - self.reserve if needed <= limit - self.reserve else limit
@@ -317,6 +333,12 @@ class _MDNeighbors:
                     f"a brick's halo needs {self.halo_needed} atoms but holds "
                     f"{self.halo_capacity}; raise capacity_multiplier",
                 )
+            if flags & _CELL_SHRANK:
+                causes.append(
+                    "the cell shrank below what the simulation's sub-bin grid covers (more "
+                    f"than about {NPT_GRID_MARGIN:.0%} in length); continue from state() in a "
+                    "new simulation",
+                )
             raise ValueError("; ".join(causes))
 
     @property
@@ -368,6 +390,27 @@ def _check_frame(model, positions, velocities, masses, cell) -> None:
             )
 
 
+def _check_npt_settings(temperature, pressure, thermostat_timescale, barostat_timescale) -> None:
+    """``md/_ensembles/_npt.py:40-56``: temperature and timescales finite and positive, the
+    pressure finite."""
+    for name, value in (("temperature", temperature), ("thermostat_timescale", thermostat_timescale),
+                        ("barostat_timescale", barostat_timescale)):
+        _positive(name, value)
+    if isinstance(pressure, bool) or not isinstance(pressure, Real) or not math.isfinite(pressure):
+        raise ValueError("pressure must be a finite number")
+
+
+def _check_barostat(barostat, device) -> None:
+    momentum = barostat.momentum if isinstance(barostat, BarostatState) else None
+    if (not isinstance(momentum, Tensor) or momentum.shape != () or momentum.dtype != torch.float64
+            or momentum.device != device):
+        raise ValueError(f"barostat.momentum must be a [] float64 tensor on {device}")
+    for field, value in zip(("positions", "momenta"), barostat.chain):
+        if (not isinstance(value, Tensor) or value.shape != (3,) or value.dtype != torch.float64
+                or value.device != device):
+            raise ValueError(f"barostat.chain.{field} must be a [3] float64 tensor on {device}")
+
+
 def _check_chain(chain, device) -> None:
     for field, value in zip(("positions", "momenta"), chain):
         if not isinstance(value, Tensor) or value.shape != (3,) or value.dtype != torch.float64 or value.device != device:
@@ -391,9 +434,15 @@ class Simulation:
                  images: Tensor | None, step: Integral, temperature: Real | None = None,
                  thermostat_timescale: Real | None = None, thermostat: ChainState | None = None,
                  virial_every_step: bool = False, force_group: int | None = None,
-                 inner_skin: Real | None | str = "auto", graphs: bool = True):
+                 inner_skin: Real | None | str = "auto", graphs: bool = True,
+                 pressure: Real | None = None, barostat_timescale: Real | None = None,
+                 barostat: BarostatState | None = None):
         if not isinstance(model, Potential):
             raise TypeError("model must be a Potential")
+        if not model._fused:
+            raise TypeError(
+                "the MD step evaluates a LennardJones or Tabulated model with the library's "
+                "kernels; a sum of models or another energy is not supported here")
         num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
         _check_frame(model, positions, velocities, masses, cell)
         if images is None:
@@ -408,6 +457,11 @@ class Simulation:
         if not bool((torch.isfinite(masses) & (masses > 0)).all()):
             raise ValueError("masses must be finite and positive")
         self._nvt = temperature is not None
+        self._npt = pressure is not None
+        if self._npt:
+            _check_npt_settings(temperature, pressure, thermostat_timescale, barostat_timescale)
+            if num_atoms < 2:
+                raise ValueError("NPT needs at least 2 atoms, for 3N - 3 degrees of freedom")
         if self._nvt:
             _positive("temperature", temperature)
             _positive("thermostat_timescale", thermostat_timescale)
@@ -426,7 +480,8 @@ class Simulation:
         self._dtype, self._device = dtype, device
         self._num_atoms = num_atoms
         self._degrees_of_freedom = 3 * num_atoms - 3
-        self._virial_every_step = bool(virial_every_step)
+        # NPT drives the box with the virial of every step
+        self._virial_every_step = bool(virial_every_step) or self._npt
         wide = {"dtype": torch.float64, "device": device}
         # resident tensors in slot order: their addresses are fixed for the graph
         self.positions = wrapped.contiguous()
@@ -443,7 +498,7 @@ class Simulation:
         self.thermostat[6] = 1.0
         self.thermostat[7] = 1.0
         self.kinetic = torch.zeros(1, **wide)
-        self.settings = {"ensemble": "NVT" if self._nvt else "NVE"}
+        self.settings = {"ensemble": "NPT" if self._npt else "NVT" if self._nvt else "NVE"}
         if self._nvt:
             self._kT = BOLTZMANN * float(temperature)
             tau = float(thermostat_timescale)
@@ -457,6 +512,28 @@ class Simulation:
         else:
             self._kT = 0.0
             self.chain_masses = torch.zeros(3, **wide)
+        self.barostat = None
+        self._box_mass = self._alpha = self._pressure = 0.0
+        if self._npt:
+            # md/_ensembles/_npt.py:114-147: box mass 3 (N + 1) kT tau_b^2, the barostat's chain
+            # of one degree of freedom with masses kT tau_b^2, alpha = 1 + 1/N, a master cell
+            tau_b = float(barostat_timescale)
+            self._pressure = float(pressure)
+            self._box_mass = 3 * (num_atoms + 1) * self._kT * tau_b * tau_b
+            self._alpha = 1.0 + 1.0 / num_atoms
+            self.barostat = torch.zeros(16, **wide)
+            self.barostat[_BOX_MASS:_BOX_MASS + 3] = self._kT * tau_b * tau_b
+            self.barostat[_MASTER_CELL:_MASTER_CELL + 3] = cell.double()
+            self.barostat[13] = 1.0
+            self.barostat[14] = 1.0
+            if barostat is not None:
+                _check_barostat(barostat, device)
+                self.barostat[_BOX_P] = barostat.momentum
+                self.barostat[_BOX_Q:_BOX_Q + 3] = barostat.chain.positions
+                self.barostat[_BOX_PC:_BOX_PC + 3] = barostat.chain.momenta
+            self.settings.update(pressure=self._pressure, barostat_timescale=tau_b)
+            # the pruned list's rule assumes a fixed cell between refreshes
+            inner_skin = None
         pot, self._keep = model._native()
         self._pot = pot
         types = model.atom_types if getattr(model, "_num_types", 1) > 1 else None
@@ -468,9 +545,15 @@ class Simulation:
             # set by its launches, and the refresh launch is one of them.
             inner_skin = (INNER_SKIN if num_atoms > _SMALL_SYSTEM and INNER_SKIN < float(skin)
                           else None)
+        grid = {}
+        if self._npt:
+            # wider sub-bins and more halo room: the cell may shrink
+            multiplier = float(capacity_multiplier)
+            grid = {"grid_margin": NPT_GRID_MARGIN,
+                    "halo_margin": multiplier + 4 * NPT_GRID_MARGIN}
         self._neighbors = _MDNeighbors(model.cutoff, skin, capacity_multiplier, self.positions,
                                        cell, types, inner_skin=inner_skin,
-                                       reserve=potential_reserve(model))
+                                       reserve=potential_reserve(model), **grid)
         if types is not None:
             pot.atom_types = _native.ptr(self._neighbors.buffers["types"])
         slots, coef = _native.ctypes.c_int64(), _native.ctypes.c_int64()
@@ -488,11 +571,12 @@ class Simulation:
             thermostat=_native.ptr(self.thermostat), chain_masses=_native.ptr(self.chain_masses),
             kinetic=_native.ptr(self.kinetic), partials=_native.ptr(self.partials),
             partial_slots=8, coefficients=_native.ptr(self.coefficients),
-            coefficient_bytes=self.coefficients.numel(),
+            coefficient_bytes=self.coefficients.numel(), barostat=_native.ptr(self.barostat),
         )
         self._settings = MDSettings(
             timestep=timestep, kT=self._kT, degrees_of_freedom=self._degrees_of_freedom,
             thermostat=1 if self._nvt else 0, virial=1 if self._virial_every_step else 0,
+            pressure=self._pressure, box_mass=self._box_mass, alpha=self._alpha,
         )
         self._groups = _GROUPS
         if force_group is not None and force_group not in self._groups:
@@ -519,7 +603,7 @@ class Simulation:
         largest halo would not fit in LDS."""
         nb = self._neighbors
         shapes = brick_shapes(nb.dims, self._num_atoms, self._dtype, nb.reserve)
-        margin = 1.0 + 0.5 * (nb.multiplier - 1.0)
+        margin = nb.halo_margin
         for k, shape in enumerate(shapes):
             nb.set_bricks(shape)
             self._set_partials(nb.bricks)
@@ -688,6 +772,8 @@ class Simulation:
 
     @property
     def fields(self) -> tuple[str, ...]:
+        if self._npt:
+            return FIELDS + ("volume", "conserved_energy")
         return FIELDS + (("conserved_energy",) if self._nvt else ())
 
     @property
@@ -720,18 +806,34 @@ class Simulation:
             bath = float((p.square() / (2.0 * self.chain_masses)).sum()
                          + self._degrees_of_freedom * self._kT * q[0] + self._kT * q[1:].sum())
             values["conserved_energy"] = energy + kinetic + bath
+        if self._npt:
+            # md/_ensembles/_npt.py:205-230: plus the barostat's chain (one degree of freedom),
+            # P V of the master cell and the box's kinetic energy
+            b = self.barostat.double()
+            box_q, box_p = b[_BOX_Q:_BOX_Q + 3], b[_BOX_PC:_BOX_PC + 3]
+            box_masses = b[_BOX_MASS:_BOX_MASS + 3]
+            box_bath = float((box_p.square() / (2.0 * box_masses)).sum()
+                             + self._kT * box_q[0] + self._kT * box_q[1:].sum())
+            master_volume = float(b[_MASTER_CELL:_MASTER_CELL + 3].prod())
+            values["volume"] = master_volume
+            values["conserved_energy"] += (box_bath + self._pressure * master_volume
+                                           + float(b[_BOX_P]) ** 2 / (2.0 * self._box_mass))
         return values
 
     def state(self) -> State:
         """A copy of the current frame (velocities with the pending chain factor applied)."""
         self._require()
         self._settle()
-        chain = None
+        chain = box = None
         if self._nvt:
             chain = ChainState(self.thermostat[0:3].clone(), self.thermostat[3:6].clone())
+        if self._npt:
+            b = self.barostat
+            box = BarostatState(b[_BOX_P].clone(), ChainState(b[_BOX_Q:_BOX_Q + 3].clone(),
+                                                              b[_BOX_PC:_BOX_PC + 3].clone()))
         return State(self._to_atoms(self.positions), self._to_atoms(self.velocities),
                      self._to_atoms(self.forces), self.cell.clone(), self._to_atoms(self.images),
-                     self._step, chain)
+                     self._step, chain, box)
 
     def _check(self, step: int) -> None:
         try:
@@ -769,4 +871,24 @@ def nvt(model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor,
                       thermostat=thermostat)
 
 
-__all__ = ("FIELDS", "Simulation", "nve", "nvt")
+def npt(model: Potential, positions: Tensor, velocities: Tensor, masses: Tensor, cell: Tensor, *,
+        timestep: Real, temperature: Real, pressure: Real, thermostat_timescale: Real,
+        barostat_timescale: Real, skin: Real = 0.1, capacity_multiplier: Real = 1.25,
+        images: Tensor | None = None, step: Integral = 0, thermostat: ChainState | None = None,
+        barostat: BarostatState | None = None) -> Simulation:
+    """Constant pressure and temperature by isotropic Martyna-Tobias-Klein dynamics
+    (``md/_ensembles/_npt.py:270-333``): the arguments of ``nvt`` plus ``pressure`` in
+    kJ/mol/nm^3 (``units.BAR`` converts), ``barostat_timescale`` tau_b in ps and, to continue a
+    run, the ``barostat`` state. Nose-Hoover chains of 3 act on the particles and on the
+    barostat; the box mass is 3 (N + 1) kT tau_b^2; the cell keeps its aspect ratio and scales
+    about the origin. The panel adds ``volume`` (nm^3) and ``conserved_energy``. The step's
+    sub-bins are chosen ``NPT_GRID_MARGIN`` wider than the list needs, so the cell may shrink by
+    that fraction of its first length; further, the run stops with ``ValueError``."""
+    return Simulation(model, positions, velocities, masses, cell, timestep=timestep, skin=skin,
+                      capacity_multiplier=capacity_multiplier, images=images, step=step,
+                      temperature=temperature, thermostat_timescale=thermostat_timescale,
+                      thermostat=thermostat, pressure=pressure,
+                      barostat_timescale=barostat_timescale, barostat=barostat)
+
+
+__all__ = ("FIELDS", "Simulation", "npt", "nve", "nvt")

@@ -12,10 +12,18 @@ class ChainState(NamedTuple):
     momenta: Tensor
 
 
+class BarostatState(NamedTuple):
+    """An isotropic barostat: the box momentum [] in kJ ps/mol, float64, and its chain."""
+
+    momentum: Tensor
+    chain: ChainState
+
+
 class State(NamedTuple):
     """A copy of a simulation's frame: positions [N, 3] in [0, L), velocities, forces, cell,
-    images [N, 3] int64 (unwrapped = positions + images * cell), step, and the thermostat of an
-    NVT run. Pass these back to ``nve``/``nvt`` to continue a run."""
+    images [N, 3] int64 (unwrapped = positions + images * cell), step, the thermostat of an NVT
+    or NPT run and the barostat of an NPT run. Pass these back to ``nve``/``nvt``/``npt`` to
+    continue a run."""
 
     positions: Tensor
     velocities: Tensor
@@ -24,7 +32,7 @@ class State(NamedTuple):
     images: Tensor
     step: int
     thermostat: ChainState | None = None
-    barostat: None = None
+    barostat: BarostatState | None = None
 
 
-__all__ = ("ChainState", "State")
+__all__ = ("BarostatState", "ChainState", "State")

@@ -97,6 +97,30 @@ def parameter_gradient(model, positions: Tensor, cell: Tensor, index: Tensor,
     return out
 
 
+def parameter_vjp(model, positions: Tensor, cell: Tensor, index: Tensor, grad_energy,
+                  grad_forces, grad_virial) -> Tensor:
+    """gE dE/dparameters + gF . dF/dparameters + gW : dW/dparameters in fp64, the layout of
+    ``parameter_gradient``; each upstream gradient may be None (zero)."""
+    x = positions.detach().contiguous()
+    cell = cell.detach().contiguous()
+    capacity = index.shape[1]
+    workspace, pot, keep = _workspace(model, capacity, x.device)
+    out = torch.empty(model._parameter_gradient_shape(), dtype=torch.float64, device=x.device)
+
+    def upstream(g):
+        return None if g is None else g.detach().to(x.dtype).contiguous()
+
+    ge, gf, gw = upstream(grad_energy), upstream(grad_forces), upstream(grad_virial)
+    _native.call("pair_parameter_vjp", _native.dtype_tag(x.dtype), _native.ptr(x),
+                 _native.ptr(cell), _native.ctypes.c_int64(x.shape[0]), _native.ptr(index),
+                 _native.ctypes.c_int64(capacity), _native.ctypes.byref(pot), _native.ptr(ge),
+                 _native.ptr(gf), _native.ptr(gw), _native.ptr(out),
+                 _native.ctypes.c_int64(out.numel()), _native.ptr(workspace),
+                 _native.ctypes.c_int64(workspace.numel()), _native.stream(x.device))
+    del keep
+    return out
+
+
 class _FirstOrderOnly(torch.autograd.Function):
     """Passes ``value`` through; differentiating it again raises. It marks the backward's
     results, which are exact first derivatives but carry no second-derivative graph."""
