@@ -10,9 +10,10 @@ kernel's roofline (algorithmic bytes / its event-timed average duration, against
 batched engine's frames/s against the plain per-system path (256 ragged systems), batched MD's
 atom-steps/s against the plain per-system step graphs (the serving stream's 14 NVT systems), a
 sum of two potentials' E/F/W on the 2M frame with and without an inner list, the reference's own
-benchmark (LJ NVT at 512 .. 32,768 atoms, ns/day beside its published RTX 5090 figures), the
-serving loop's p50/p99 latency and throughput against the plain per-request path, and a CPU
-baseline (the oracle's restatement of the reference path on this host's cores).
+benchmark (LJ NVT at 512 .. 32,768 atoms, ns/day beside its published RTX 5090 figures), one
+update of a 64-frame force-matching fit of a table, the serving loop's p50/p99 latency and
+throughput against the plain per-request path, and a CPU baseline (the oracle's restatement of
+the reference path on this host's cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batching]
                     [--no-serving]
@@ -292,6 +293,126 @@ def bench_sum(torch, x, cell, repeats=5):
     return out
 
 
+def bench_fitting(torch, repeats=5, cpu_frames=2):
+    """Force matching of a table: 64 frames of 4,096 beads (8 nm^-3, L = 8 nm), a one-type
+    200-knot table on [0.3, 1.2] nm (last knot 0), targets from LJ(0.47 nm, 3.5 kJ/mol) at
+    rc 1.2 nm (forces plus N(0, 5) noise; E and W exact), fp32, loss E + F + W. Times one
+    update's batch gradient over all 64 frames (HIP events, mean of ``repeats``) through
+    ``training`` (a zero-skin pair stack per frame; ndn_pair_efw forward and
+    ndn_pair_parameter_vjp backward per frame) and through ``fit.TableFrames`` (the batched
+    engine, all frames in one launch, and ndn_batch_table_vjp), and the oracle's autograd
+    double backward on ``cpu_frames`` frames on the host. Reports the loss in the reference's
+    scaled form and as plain MSE, and the two GPU routes' gradient distance."""
+    import numpy as np
+
+    from nabladyn_amd import fit, potential, training, workloads
+    from nabladyn_amd.training import _fit
+
+    device = torch.device("cuda", 0)
+    frames, knots, start, cutoff = 64, 200, 0.3, 1.2
+    xs, cells, hidden = workloads.cg_frames(frames=frames, beads=4096)
+    beads = xs.shape[1]
+    positions64 = torch.tensor(xs, dtype=torch.float64, device=device)
+    cells64 = torch.tensor(cells, dtype=torch.float64, device=device)
+    types = torch.zeros(beads, dtype=torch.int64, device=device)
+    truth = potential.lennard_jones(
+        atom_types=types, sigma=torch.tensor([hidden["sigma"]], dtype=torch.float64, device=device),
+        epsilon=torch.tensor([hidden["epsilon"]], dtype=torch.float64, device=device),
+        cutoff=cutoff)
+    stack64 = training.pair_stack(positions64, cells64, cutoff)
+    with torch.no_grad():
+        labels = training.predict(truth, {}, positions64, cells64, (stack64,))
+    noise = hidden["rng"].normal(0.0, hidden["force_noise"], (frames, beads, 3))
+    labels["forces"] = labels["forces"] + torch.tensor(noise, device=device)
+    labels = {k: v.float() for k, v in labels.items()}
+    r = start + (cutoff - start) / (knots - 1) * np.arange(knots)
+    s6 = (0.45 / r) ** 6
+    table = np.clip(4 * 3.0 * (s6 * s6 - s6), -50, 50)
+    table[-1] = 0.0
+    model = potential.tabulated(
+        atom_types=types, table=torch.tensor(table[None], dtype=torch.float32, device=device),
+        start=torch.tensor([start], dtype=torch.float32, device=device), cutoff=cutoff,
+        trainable=("table",))
+    weights = {"energy": 1.0, "forces": 1.0, "virial": 1.0}
+    data = training.Dataset(xs.astype(np.float32), cells.astype(np.float32),
+                            **{k: v.cpu() for k, v in labels.items()})
+    t0 = time.perf_counter()
+    prepared = _fit._prepare(model, data, training.CHANNELS)
+    torch.cuda.synchronize()
+    prepare_s = time.perf_counter() - t0
+    scales = training.scales(**prepared.labels)
+    trained = {"table": model.table}
+    everything = torch.arange(frames)
+
+    def timed(fn):
+        fn()
+        begin, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        begin.record()
+        for _ in range(repeats):
+            result = fn()
+        end.record()
+        torch.cuda.synchronize()
+        return begin.elapsed_time(end) / repeats, result
+
+    per_frame_ms, (gradient, scaled, _) = timed(lambda: _fit.batch_gradient(
+        model, trained, prepared, everything, frames, weights, scales))
+    batched = fit.TableFrames(model, prepared.positions, prepared.cells)
+
+    def batched_gradient():
+        t = model.table.detach().clone().requires_grad_(True)
+        value, _ = fit.loss(batched.predict(t), prepared.labels, weights, scales)
+        (g,) = torch.autograd.grad(value, t)
+        return g
+
+    batched_ms, batched_grad = timed(batched_gradient)
+    with torch.no_grad():
+        predicted = training.predict(model, {}, prepared.positions, prepared.cells,
+                                     prepared.stacks)
+    plain = {k: float((predicted[k].double() - prepared.labels[k].double()).square().mean())
+             for k in training.CHANNELS}
+    g = gradient["table"].double()
+    distance = float((g - batched_grad.double()).abs().max() / g.abs().max())
+    # the oracle's double backward (the reference's route) on a few frames, on the host
+    from oracle import neighbors as on
+    from oracle import potentials as op
+    from oracle import training as ot
+
+    sample = []
+    for f in range(cpu_frames):
+        xw = on.wrap(xs[f].astype(np.float32).astype(np.float64), cells[f])[0]
+        built = on.build(xw, cells[f], cutoff)
+        index = torch.tensor(built.pairs[:, :built.pair_total]).long()
+        sample.append((torch.tensor(xw, dtype=torch.float32),
+                       torch.tensor(cells[f], dtype=torch.float32), index))
+    spacing = torch.tensor([(cutoff - start) / (knots - 1)])
+    cpu_types = torch.zeros(beads, dtype=torch.int64)
+
+    def energy(p, x, strain, cell, index):
+        return op.t_table_energy(x, strain, cell, index, cpu_types, p[0],
+                                 torch.tensor([start]), spacing, cutoff)
+
+    cpu_labels = {k: v[:cpu_frames].cpu() for k, v in prepared.labels.items()}
+    t0 = time.perf_counter()
+    ot.loss_and_gradient(energy, [torch.tensor(table[None], dtype=torch.float32)], sample,
+                         cpu_labels, weights, scales)
+    cpu_ms = (time.perf_counter() - t0) * 1e3 / cpu_frames
+    return {
+        "workload": "64 frames x 4,096 beads, one-type 200-knot table, E+F+W, fp32",
+        "pairs_columns": int(prepared.stacks[0].shape[2]),
+        "prepare_s": prepare_s,
+        "update_ms_training": per_frame_ms,
+        "frames_per_s_training": frames / per_frame_ms * 1e3,
+        "update_ms_batched_engine": batched_ms,
+        "frames_per_s_batched_engine": frames / batched_ms * 1e3,
+        "gradient_distance_between_routes": distance,
+        "loss_scaled": float(scaled),
+        "loss_plain_mse": plain,
+        "cpu_oracle_ms_per_frame": cpu_ms,
+        "cpu_cores": torch.get_num_threads(),
+    }
+
+
 # The reference's published LJ NVT figures (1x RTX 5090, fp32, dt 4 fs, skin 0.1 nm, rc 1.0 nm,
 # Nose-Hoover chain of 3 at 120 K; README.md:594-603): atoms -> ns/day.
 REFERENCE_NS_PER_DAY = {512: 9288.0, 4096: 5507.0, 13824: 3632.0, 32768: 2210.0}
@@ -396,14 +517,15 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batching", action="store_true")
     parser.add_argument(
-        "--kernel", choices=("force", "copy", "md-batch", "sum", "published"),
+        "--kernel", choices=("force", "copy", "md-batch", "sum", "published", "fitting"),
         help="only one measurement, for counter collection and traces: 'force' builds the "
              "2M-atom system (4 lanes per row, the autotuner's choice for it), runs 100 steps, "
              "then launches its force kernel 23 times (the last 23 dispatches of that kernel); 'copy' copies 1 GiB on the device 20 times (the "
              "scale of the traffic counters); 'md-batch' steps the serving stream's 14 MD "
              "systems in batches (20 + 200 steps) and prints that leg alone; 'sum' evaluates "
              "potential.sum on the 2M frame and prints that leg alone; 'published' runs the "
-             "reference's published sizes and prints that leg alone")
+             "reference's published sizes and prints that leg alone; 'fitting' times one "
+             "update of the 64-frame table fit and prints that leg alone")
     parser.add_argument("--no-serving", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -433,6 +555,9 @@ def main() -> int:
         return 0
     if args.kernel == "published":
         print(json.dumps({"kernel": "published", "sizes": bench_published_sizes(torch)}))
+        return 0
+    if args.kernel == "fitting":
+        print(json.dumps({"kernel": "fitting", **bench_fitting(torch)}))
         return 0
     if args.kernel == "sum":
         from nabladyn_amd import workloads
@@ -572,6 +697,8 @@ def main() -> int:
         result["sum"] = bench_sum(torch, x, cell)
         log("the reference's published sizes (512 .. 32,768 atoms) ...")
         result["published_sizes"] = bench_published_sizes(torch)
+        log("fitting: 64 frames of 4,096 beads, a 200-knot table ...")
+        result["fitting"] = bench_fitting(torch)
     if not args.no_serving:
         log("serving: a Poisson request stream ...")
         result["serving"] = bench_serving(torch)

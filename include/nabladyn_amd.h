@@ -188,6 +188,33 @@ int ndn_pair_parameter_vjp(int dtype, const void* positions, const void* cell, i
                            const void* grad_virial, double* out, int64_t out_count,
                            void* workspace, int64_t workspace_bytes, ndn_stream_t stream);
 
+/* Workspace bytes of ndn_pair_efw_frames (and enough for ndn_pair_parameter_vjp_frames) for F
+ * frames (1 <= F <= 65535) of `columns` columns each. */
+int ndn_pair_frames_workspace_bytes(int dtype, const ndn_potential* pot, int64_t frames,
+                                    int64_t columns, int64_t* bytes);
+
+/* ndn_pair_efw over F frames of one system in one launch: wrapped positions [F, N, 3], cells
+ * [F, 3], pair stacks [F, 2, C] (each row a half list padded with self-pairs, as the fit's
+ * pair_stack writes it); energy [F], forces [F, N, 3], virial [F, 3, 3], any may be NULL. Each
+ * frame's results equal ndn_pair_efw's on that frame. Replaces torch.func.vmap of the per-frame
+ * channels in the reference's fit (training/_loss.py:71-102). The workspace is
+ * ndn_pair_frames_workspace_bytes. */
+int ndn_pair_efw_frames(int dtype, const void* positions, const void* cells, int64_t frames,
+                        int64_t num_atoms, const int32_t* stacks, int64_t capacity,
+                        const ndn_potential* pot, void* energy, void* forces, void* virial,
+                        void* workspace, int64_t workspace_bytes, ndn_stream_t stream);
+
+/* ndn_pair_parameter_vjp over the F frames of ndn_pair_efw_frames, summed into one out: upstream
+ * grad_energy [F], grad_forces [F, N, 3], grad_virial [F, 3, 3] (any may be NULL). Replaces the
+ * vmapped nested grad of training/_loss.py:71-125 with training/_fit.py:245-271. The workspace
+ * is ndn_pair_workspace_bytes of C columns (or the frames' size). */
+int ndn_pair_parameter_vjp_frames(int dtype, const void* positions, const void* cells,
+                                  int64_t frames, int64_t num_atoms, const int32_t* stacks,
+                                  int64_t capacity, const ndn_potential* pot,
+                                  const void* grad_energy, const void* grad_forces,
+                                  const void* grad_virial, double* out, int64_t out_count,
+                                  void* workspace, int64_t workspace_bytes, ndn_stream_t stream);
+
 /* ndn_pair_efw without the workspace argument, for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
  * counterpart in the reference. Its scratch (the potential's coefficients, then 64 bytes per

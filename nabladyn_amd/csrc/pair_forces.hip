@@ -92,6 +92,9 @@ __device__ __forceinline__ T run_sum(T value, int lane, int start)
     return value;
 }
 
+//
+// Frames of one system (training stacks) take blockIdx.y as the frame: its positions [N, 3],
+// cell [3], pairs [2, C], forces [N, 3] and the grid's partial slots follow the previous frame's.
 template <typename T, class Pot, bool kForces, bool kVirial>
 __global__ void __launch_bounds__(kBlock) half_list_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell,
@@ -99,6 +102,12 @@ __global__ void __launch_bounds__(kBlock) half_list_kernel(
     T cutoff_squared, Pot pot, T* __restrict__ forces, double* __restrict__ partials)
 {
     __shared__ double shared[16];
+    const std::int64_t frame = blockIdx.y;
+    positions += 3 * frame * num_atoms;
+    cell += 3 * frame;
+    pairs += 2 * frame * capacity;
+    if (kForces) forces += 3 * frame * num_atoms;
+    partials += 8 * frame * gridDim.x;
     const T lx = cell[0], ly = cell[1], lz = cell[2];
     const int lane = lane_id();
     T energy = 0;
@@ -267,13 +276,18 @@ __global__ void __launch_bounds__(kBlock) full_rows_kernel(
 // ---- final reduction ------------------------------------------------------------------------
 
 // Sums the per-block partials (8 doubles each: energy, then xx yy zz xy xz yz) in a fixed order
-// and writes E [1] and W [3, 3] (symmetric) in T, each when its pointer is given.
+// and writes E [1] and W [3, 3] (symmetric) in T, each when its pointer is given. Block f sums
+// frame f's `blocks` slots into E [f] and W [f].
 template <typename T>
 __global__ void __launch_bounds__(kBlock) finish_kernel(const double* __restrict__ partials,
                                                         int blocks, T* energy, T* virial,
                                                         double scale)
 {
     __shared__ double shared[16];
+    const std::int64_t frame = blockIdx.x;
+    partials += 8 * frame * blocks;
+    if (energy) energy += frame;
+    if (virial) virial += 9 * frame;
     double sums[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < blocks; b += blockDim.x)
         for (int k = 0; k < 7; ++k) sums[k] += partials[8LL * b + k];
@@ -371,6 +385,7 @@ template int prepare_potential<double>(const ndn_potential*, void*, hipStream_t,
                                        PotentialView<double>*);
 
 constexpr int kHalfBlocks = 2048;
+constexpr std::int64_t kMaxFrames = 65535;  // the grid's y extent
 
 int half_blocks(std::int64_t capacity)
 {
@@ -395,8 +410,9 @@ template int finish<double>(const double*, int, void*, void*, double, hipStream_
 template <typename T, class Pot>
 void launch_half(const T* x, const T* cell, const index_t* pairs, std::int64_t capacity,
                  index_t num_atoms, T cutoff_squared, const Pot& pot, T* forces, bool virial,
-                 double* partials, int blocks, hipStream_t stream)
+                 double* partials, int columns_blocks, hipStream_t stream, int frames = 1)
 {
+    const dim3 blocks(static_cast<unsigned>(columns_blocks), static_cast<unsigned>(frames));
     if (forces && virial)
         half_list_kernel<T, Pot, true, true><<<blocks, kBlock, 0, stream>>>(
             x, cell, pairs, capacity, num_atoms, cutoff_squared, pot, forces, partials);
@@ -500,6 +516,45 @@ int pair_efw_impl(const void* positions, const void* cell, std::int64_t num_atom
     return finish<T>(partials, blocks, energy, virial, 1.0, stream);
 }
 
+// Frames of one system over their pair stacks [F, 2, C]: one launch of F x blocks, one finish
+// block per frame.
+template <typename T>
+int pair_efw_frames_impl(const void* positions, const void* cells, std::int64_t frames,
+                         std::int64_t num_atoms, const index_t* stacks, std::int64_t capacity,
+                         const ndn_potential* pot, void* energy, void* forces, void* virial,
+                         void* workspace, hipStream_t stream)
+{
+    PotentialView<T> view;
+    const int prepared = prepare_potential<T>(pot, workspace, stream, &view);
+    if (prepared != NDN_OK) return prepared;
+    const int blocks = half_blocks(capacity);
+    auto* partials = reinterpret_cast<double*>(static_cast<std::uint8_t*>(workspace) +
+                                               coefficient_bytes(pot, sizeof(T)));
+    T* f = static_cast<T*>(forces);
+    if (f != nullptr)
+    {
+        zero_kernel<T><<<std::min<unsigned>(blocks_for(3 * frames * num_atoms), 4096u), kBlock,
+                         0, stream>>>(f, 3 * frames * num_atoms);
+        NDN_LAUNCHED("zero_forces");
+    }
+    const T* x = static_cast<const T*>(positions);
+    const T* L = static_cast<const T*>(cells);
+    const index_t n = static_cast<index_t>(num_atoms);
+    const int count = static_cast<int>(frames);
+    if (view.kind == NDN_LJ)
+        launch_half<T>(x, L, stacks, capacity, n, view.cutoff_squared, view.lj, f,
+                       virial != nullptr, partials, blocks, stream, count);
+    else
+        launch_half<T>(x, L, stacks, capacity, n, view.cutoff_squared, view.table, f,
+                       virial != nullptr, partials, blocks, stream, count);
+    NDN_LAUNCHED("half_list_frames");
+    if (energy == nullptr && virial == nullptr) return NDN_OK;
+    finish_kernel<T><<<count, kBlock, 0, stream>>>(partials, blocks, static_cast<T*>(energy),
+                                                   static_cast<T*>(virial), 1.0);
+    NDN_LAUNCHED("finish_frames");
+    return NDN_OK;
+}
+
 // ---- the entry point that allocates nothing --------------------------------------------------
 
 // The scratch of ndn_pair_efw_noalloc: the potential's coefficients from byte 0, then one
@@ -562,6 +617,43 @@ int ndn_pair_workspace_bytes(int dtype, const ndn_potential* pot, int64_t column
     const std::int64_t blocks = std::max<std::int64_t>(half_blocks(columns), kPartialSlots);
     *bytes = coefficient_bytes(pot, element) + blocks * 8 * static_cast<std::int64_t>(sizeof(double));
     return NDN_OK;
+}
+
+int ndn_pair_frames_workspace_bytes(int dtype, const ndn_potential* pot, int64_t frames,
+                                    int64_t columns, int64_t* bytes)
+{
+    const int checked = check_potential(pot, 0);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    NDN_REQUIRE(frames >= 1 && frames <= kMaxFrames && columns >= 0 && bytes != nullptr,
+                "frames must lie in [1, 65535] and columns be nonnegative");
+    const int element = dtype == NDN_F32 ? 4 : 8;
+    const std::int64_t slots = frames * half_blocks(columns);
+    *bytes = coefficient_bytes(pot, element) +
+             std::max<std::int64_t>(slots, kPartialSlots) * 8 *
+                 static_cast<std::int64_t>(sizeof(double));
+    return NDN_OK;
+}
+
+int ndn_pair_efw_frames(int dtype, const void* positions, const void* cells, int64_t frames,
+                        int64_t num_atoms, const int32_t* stacks, int64_t capacity,
+                        const ndn_potential* pot, void* energy, void* forces, void* virial,
+                        void* workspace, int64_t workspace_bytes, ndn_stream_t stream)
+{
+    const int checked = check_potential(pot, num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(positions && cells && stacks && workspace, "a buffer is null");
+    NDN_REQUIRE(num_atoms >= 1 && num_atoms <= INT32_MAX, "N must satisfy 1 <= N < 2**31");
+    NDN_REQUIRE(capacity >= 1 && capacity <= INT32_MAX, "stacks must be [F, 2, C] with int32 C");
+    std::int64_t needed = 0;
+    const int sized = ndn_pair_frames_workspace_bytes(dtype, pot, frames, capacity, &needed);
+    if (sized != NDN_OK) return sized;
+    NDN_REQUIRE(workspace_bytes >= needed, "the workspace is too small");
+    if (dtype == NDN_F32)
+        return pair_efw_frames_impl<float>(positions, cells, frames, num_atoms, stacks, capacity,
+                                           pot, energy, forces, virial, workspace, stream);
+    return pair_efw_frames_impl<double>(positions, cells, frames, num_atoms, stacks, capacity,
+                                        pot, energy, forces, virial, workspace, stream);
 }
 
 int ndn_pair_efw(int dtype, const void* positions, const void* cell, int64_t num_atoms,

@@ -76,6 +76,15 @@ struct Upstream
     const T* virial;   // [3, 3] or null
     bool plain;        // dE/dtheta: gE = 1, no F or W term
 
+    // frame f of frames stacked one after another: gE [F], gF [F, N, 3], gW [F, 3, 3]
+    __device__ __forceinline__ Upstream at(std::int64_t frame, index_t num_atoms) const
+    {
+        if (plain || frame == 0) return *this;
+        return Upstream{energy ? energy + frame : nullptr,
+                        forces ? forces + 3 * frame * num_atoms : nullptr,
+                        virial ? virial + 9 * frame : nullptr, false};
+    }
+
     // gE and the slope weight (gF_i - gF_j) . d + d^T gW d of a pair
     __device__ __forceinline__ void weights(index_t i, index_t j, const T d[3], double& ge,
                                             double& slope) const
@@ -217,10 +226,16 @@ __global__ void __launch_bounds__(kBlock) lj_parameter_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell, const index_t* __restrict__ pairs,
     std::int64_t capacity, index_t num_atoms, T cutoff_squared, const T* __restrict__ sigma,
     const T* __restrict__ epsilon, const std::int64_t* __restrict__ types, int num_types,
-    const T* __restrict__ strain_matrix, Upstream<T> up, double* __restrict__ out)
+    const T* __restrict__ strain_matrix, Upstream<T> frames_up, double* __restrict__ out)
 {
     __shared__ double local_store[kLocalGradient];
     const Strain<T> strain = read_strain(strain_matrix);
+    // frames of one system (blockIdx.y) add into the same out
+    const std::int64_t frame = blockIdx.y;
+    positions += 3 * frame * num_atoms;
+    cell += 3 * frame;
+    pairs += 2 * frame * capacity;
+    const Upstream<T> up = frames_up.at(frame, num_atoms);
     const int rows = num_types * (num_types + 1) / 2;
     double* local = 2 * rows <= kLocalGradient ? local_store : nullptr;
     if (local != nullptr)
@@ -271,10 +286,15 @@ template <typename T, bool kStrained>
 __global__ void __launch_bounds__(kBlock) table_parameter_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell, const index_t* __restrict__ pairs,
     std::int64_t capacity, index_t num_atoms, T cutoff_squared, Table<T> table,
-    const T* __restrict__ strain_matrix, Upstream<T> up, double* __restrict__ out)
+    const T* __restrict__ strain_matrix, Upstream<T> frames_up, double* __restrict__ out)
 {
     __shared__ double local_store[kLocalGradient];
     const Strain<T> strain = read_strain(strain_matrix);
+    const std::int64_t frame = blockIdx.y;
+    positions += 3 * frame * num_atoms;
+    cell += 3 * frame;
+    pairs += 2 * frame * capacity;
+    const Upstream<T> up = frames_up.at(frame, num_atoms);
     const int knots = table.knots;
     const int rows = table.num_types * (table.num_types + 1) / 2;
     const std::int64_t entries = static_cast<std::int64_t>(rows) * knots;
@@ -413,7 +433,8 @@ template <typename T>
 int parameter_impl(const void* positions, const void* cell, std::int64_t num_atoms,
                    const index_t* pairs, std::int64_t capacity, const ndn_potential* pot,
                    const void* strain, const Upstream<T>& up, double* out,
-                   std::int64_t out_count, void* workspace, hipStream_t stream)
+                   std::int64_t out_count, void* workspace, hipStream_t stream,
+                   std::int64_t frames = 1)
 {
     PotentialView<T> view;
     const int prepared = prepare_potential<T>(pot, workspace, stream, &view);
@@ -421,7 +442,8 @@ int parameter_impl(const void* positions, const void* cell, std::int64_t num_ato
     zero_values<double><<<std::min<unsigned>(blocks_for(out_count), 256u), kBlock, 0, stream>>>(
         out, out_count);
     NDN_LAUNCHED("zero_gradient");
-    const int blocks = gradient_blocks(capacity);
+    const dim3 blocks(static_cast<unsigned>(gradient_blocks(capacity)),
+                      static_cast<unsigned>(frames));
     const T* x = static_cast<const T*>(positions);
     const T* L = static_cast<const T*>(cell);
     const index_t n = static_cast<index_t>(num_atoms);
@@ -465,6 +487,32 @@ int check_pair_call(int dtype, const ndn_potential* pot, const void* positions, 
     if (sized != NDN_OK) return sized;
     NDN_REQUIRE(workspace_bytes >= needed, "the workspace is too small");
     return NDN_OK;
+}
+
+// The VJP entry points' shared tail: out's size, then the launch over `frames` frames.
+int vjp_call(int dtype, const void* positions, const void* cells, std::int64_t frames,
+             std::int64_t num_atoms, const int32_t* pairs, std::int64_t capacity,
+             const ndn_potential* pot, const void* grad_energy, const void* grad_forces,
+             const void* grad_virial, double* out, std::int64_t out_count, void* workspace,
+             hipStream_t stream)
+{
+    const std::int64_t rows = static_cast<std::int64_t>(pot->num_types) * (pot->num_types + 1) / 2;
+    const std::int64_t expected = pot->kind == NDN_LJ ? 2 * rows : rows * pot->knots;
+    NDN_REQUIRE(out != nullptr && out_count == expected,
+                "out must hold 2P doubles for LJ (dsigma, depsilon) or P * K for a table");
+    if (dtype == NDN_F32)
+        return parameter_impl<float>(
+            positions, cells, num_atoms, pairs, capacity, pot, nullptr,
+            Upstream<float>{static_cast<const float*>(grad_energy),
+                            static_cast<const float*>(grad_forces),
+                            static_cast<const float*>(grad_virial), false},
+            out, out_count, workspace, stream, frames);
+    return parameter_impl<double>(
+        positions, cells, num_atoms, pairs, capacity, pot, nullptr,
+        Upstream<double>{static_cast<const double*>(grad_energy),
+                         static_cast<const double*>(grad_forces),
+                         static_cast<const double*>(grad_virial), false},
+        out, out_count, workspace, stream, frames);
 }
 
 }  // namespace ndn
@@ -520,23 +568,23 @@ int ndn_pair_parameter_vjp(int dtype, const void* positions, const void* cell, i
     const int checked = check_pair_call(dtype, pot, positions, cell, num_atoms, pairs, capacity,
                                         workspace, workspace_bytes);
     if (checked != NDN_OK) return checked;
-    const std::int64_t rows = static_cast<std::int64_t>(pot->num_types) * (pot->num_types + 1) / 2;
-    const std::int64_t expected = pot->kind == NDN_LJ ? 2 * rows : rows * pot->knots;
-    NDN_REQUIRE(out != nullptr && out_count == expected,
-                "out must hold 2P doubles for LJ (dsigma, depsilon) or P * K for a table");
-    if (dtype == NDN_F32)
-        return parameter_impl<float>(
-            positions, cell, num_atoms, pairs, capacity, pot, nullptr,
-            Upstream<float>{static_cast<const float*>(grad_energy),
-                            static_cast<const float*>(grad_forces),
-                            static_cast<const float*>(grad_virial), false},
-            out, out_count, workspace, stream);
-    return parameter_impl<double>(
-        positions, cell, num_atoms, pairs, capacity, pot, nullptr,
-        Upstream<double>{static_cast<const double*>(grad_energy),
-                         static_cast<const double*>(grad_forces),
-                         static_cast<const double*>(grad_virial), false},
-        out, out_count, workspace, stream);
+    return vjp_call(dtype, positions, cell, 1, num_atoms, pairs, capacity, pot, grad_energy,
+                    grad_forces, grad_virial, out, out_count, workspace, stream);
+}
+
+int ndn_pair_parameter_vjp_frames(int dtype, const void* positions, const void* cells,
+                                  int64_t frames, int64_t num_atoms, const int32_t* stacks,
+                                  int64_t capacity, const ndn_potential* pot,
+                                  const void* grad_energy, const void* grad_forces,
+                                  const void* grad_virial, double* out, int64_t out_count,
+                                  void* workspace, int64_t workspace_bytes, ndn_stream_t stream)
+{
+    const int checked = check_pair_call(dtype, pot, positions, cells, num_atoms, stacks, capacity,
+                                        workspace, workspace_bytes);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(frames >= 1 && frames <= 65535, "frames must lie in [1, 65535]");
+    return vjp_call(dtype, positions, cells, frames, num_atoms, stacks, capacity, pot,
+                    grad_energy, grad_forces, grad_virial, out, out_count, workspace, stream);
 }
 
 }  // extern "C"

@@ -10,7 +10,7 @@ reference's eager energies (``potential/_base.py:82-106``, ``quantity.py:93-117`
 
 Limits, stated where they apply:
 - first derivatives only: a derivative of the backward's results (forces in a loss) raises;
-  force and virial fitting has its own hand-written backward (``nabladyn_amd.fit``);
+  force and virial fitting has its own hand-written backward (``nabladyn_amd.training``);
 - no derivative with respect to the cell (the reference's is that of its minimum-image
   shifts); the cell's gradient is None;
 - ``vmap`` over frames runs one launch per frame.
@@ -23,8 +23,8 @@ from .. import _native
 from ..space import frames as _frames
 
 
-def _workspace(model, capacity: int, device):
-    pot, keep = model._native()
+def _workspace(model, capacity: int, device, native=None):
+    pot, keep = model._native() if native is None else native
     size = _native.ctypes.c_int64()
     _native.call("pair_workspace_bytes", _native.dtype_tag(model.dtype), _native.ctypes.byref(pot),
                  _native.ctypes.c_int64(capacity), _native.ctypes.byref(size))
@@ -32,14 +32,15 @@ def _workspace(model, capacity: int, device):
 
 
 def evaluate(model, positions: Tensor, cell: Tensor, index: Tensor, strain: Tensor | None,
-             energy=True, forces=True, virial=True):
+             energy=True, forces=True, virial=True, native=None):
     """E [], F [N, 3] and W [3, 3] of wrapped positions on the half list ``index`` [2, C] (int32,
-    contiguous), at ``strain`` or none, from one launch; None for what is not asked."""
+    contiguous), at ``strain`` or none, from one launch; None for what is not asked. ``native``
+    is a description from ``model._native()`` taken earlier, else the model's own now."""
     x = positions.detach().contiguous()
     cell = cell.detach().contiguous()
     dtype, device = x.dtype, x.device
     capacity = index.shape[1]
-    workspace, pot, keep = _workspace(model, capacity, device)
+    workspace, pot, keep = _workspace(model, capacity, device, native)
     e = torch.empty((), dtype=dtype, device=device) if energy else None
     f = torch.empty((x.shape[0], 3), dtype=dtype, device=device) if forces else None
     w = torch.empty((3, 3), dtype=dtype, device=device) if virial else None
@@ -98,13 +99,14 @@ def parameter_gradient(model, positions: Tensor, cell: Tensor, index: Tensor,
 
 
 def parameter_vjp(model, positions: Tensor, cell: Tensor, index: Tensor, grad_energy,
-                  grad_forces, grad_virial) -> Tensor:
+                  grad_forces, grad_virial, native=None) -> Tensor:
     """gE dE/dparameters + gF . dF/dparameters + gW : dW/dparameters in fp64, the layout of
-    ``parameter_gradient``; each upstream gradient may be None (zero)."""
+    ``parameter_gradient``; each upstream gradient may be None (zero). ``native`` as for
+    ``evaluate``."""
     x = positions.detach().contiguous()
     cell = cell.detach().contiguous()
     capacity = index.shape[1]
-    workspace, pot, keep = _workspace(model, capacity, x.device)
+    workspace, pot, keep = _workspace(model, capacity, x.device, native)
     out = torch.empty(model._parameter_gradient_shape(), dtype=torch.float64, device=x.device)
 
     def upstream(g):
@@ -116,6 +118,61 @@ def parameter_vjp(model, positions: Tensor, cell: Tensor, index: Tensor, grad_en
                  _native.ctypes.c_int64(capacity), _native.ctypes.byref(pot), _native.ptr(ge),
                  _native.ptr(gf), _native.ptr(gw), _native.ptr(out),
                  _native.ctypes.c_int64(out.numel()), _native.ptr(workspace),
+                 _native.ctypes.c_int64(workspace.numel()), _native.stream(x.device))
+    del keep
+    return out
+
+
+def _frames_workspace(model, frames: int, capacity: int, device, native):
+    pot, keep = native
+    size = _native.ctypes.c_int64()
+    _native.call("pair_frames_workspace_bytes", _native.dtype_tag(model.dtype),
+                 _native.ctypes.byref(pot), _native.ctypes.c_int64(frames),
+                 _native.ctypes.c_int64(capacity), _native.ctypes.byref(size))
+    return torch.empty(size.value, dtype=torch.uint8, device=device), pot, keep
+
+
+def evaluate_frames(model, positions: Tensor, cells: Tensor, stacks: Tensor, native):
+    """E [F], F [F, N, 3] and W [F, 3, 3] of F frames of wrapped positions [F, N, 3] in cells
+    [F, 3] over pair stacks [F, 2, C] (int32, contiguous), from one launch; ``native`` is
+    ``model._native()``."""
+    x = positions.detach().contiguous()
+    cells = cells.detach().contiguous()
+    frames, atoms = x.shape[0], x.shape[1]
+    dtype, device = x.dtype, x.device
+    workspace, pot, keep = _frames_workspace(model, frames, stacks.shape[2], device, native)
+    e = torch.empty(frames, dtype=dtype, device=device)
+    f = torch.empty((frames, atoms, 3), dtype=dtype, device=device)
+    w = torch.empty((frames, 3, 3), dtype=dtype, device=device)
+    _native.call("pair_efw_frames", _native.dtype_tag(dtype), _native.ptr(x), _native.ptr(cells),
+                 _native.ctypes.c_int64(frames), _native.ctypes.c_int64(atoms),
+                 _native.ptr(stacks), _native.ctypes.c_int64(stacks.shape[2]),
+                 _native.ctypes.byref(pot), _native.ptr(e), _native.ptr(f), _native.ptr(w),
+                 _native.ptr(workspace), _native.ctypes.c_int64(workspace.numel()),
+                 _native.stream(device))
+    del keep
+    return e, f, w
+
+
+def parameter_vjp_frames(model, positions: Tensor, cells: Tensor, stacks: Tensor, grad_energy,
+                         grad_forces, grad_virial, native) -> Tensor:
+    """``parameter_vjp`` summed over the frames of ``evaluate_frames``; upstream gradients
+    [F], [F, N, 3], [F, 3, 3], each may be None."""
+    x = positions.detach().contiguous()
+    cells = cells.detach().contiguous()
+    frames, atoms = x.shape[0], x.shape[1]
+    workspace, pot, keep = _frames_workspace(model, frames, stacks.shape[2], x.device, native)
+    out = torch.empty(model._parameter_gradient_shape(), dtype=torch.float64, device=x.device)
+
+    def upstream(g):
+        return None if g is None else g.detach().to(x.dtype).contiguous()
+
+    ge, gf, gw = upstream(grad_energy), upstream(grad_forces), upstream(grad_virial)
+    _native.call("pair_parameter_vjp_frames", _native.dtype_tag(x.dtype), _native.ptr(x),
+                 _native.ptr(cells), _native.ctypes.c_int64(frames), _native.ctypes.c_int64(atoms),
+                 _native.ptr(stacks), _native.ctypes.c_int64(stacks.shape[2]),
+                 _native.ctypes.byref(pot), _native.ptr(ge), _native.ptr(gf), _native.ptr(gw),
+                 _native.ptr(out), _native.ctypes.c_int64(out.numel()), _native.ptr(workspace),
                  _native.ctypes.c_int64(workspace.numel()), _native.stream(x.device))
     del keep
     return out
@@ -139,8 +196,8 @@ class _FirstOrderOnly(torch.autograd.Function):
     def backward(ctx, grad):
         raise RuntimeError(
             "second derivatives of the LJ and tabulated energies are not available (the fused "
-            "kernel returns F and W, not their derivatives); force and virial fitting of a "
-            "table uses nabladyn_amd.fit")
+            "kernel returns F and W, not their derivatives); force and virial fitting uses "
+            "nabladyn_amd.training, whose backward is the parameter VJP kernel")
 
 
 class _PairEnergy(torch.autograd.Function):

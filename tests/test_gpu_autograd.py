@@ -71,7 +71,7 @@ def _oracle(model, x, cell, pairs, strain):
         e = op.t_table_energy(xw, s, c, idx, types, params[0], model.start.cpu().double(),
                               model.spacing.cpu().double(), model.cutoff)
     grads = torch.autograd.grad(e, [xw, s, *params])
-    return float(e), [g.numpy() for g in grads]
+    return float(e.detach()), [g.numpy() for g in grads]
 
 
 @pytest.mark.parametrize("kind", ["lj", "table"])
