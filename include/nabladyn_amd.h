@@ -110,7 +110,13 @@ enum { NDN_LJ = 0, NDN_TABLE = 1 };
  * (potential/_lennard_jones/_potential.py:143-151). LJ reads sigma, epsilon [P] T
  * (potential/_lennard_jones/_potential.py:113-120); a table reads table [P, K] T with its last
  * column 0, start [P] T and spacing [P] T (potential/_tabulated/_potential.py:59-91). The
- * cutoff is in nm; atom_types [N] int64 may be NULL with one type. */
+ * cutoff is in nm; atom_types [N] int64 may be NULL with one type.
+ *
+ * next: NULL, or the second model of a sum of two (potential.sum, potential/_sum/_potential.py:
+ * 86-99), for the ndn_md_* entry points only; every other entry point refuses it. The two share
+ * num_types and atom_types, and the MD step evaluates both in one walk of its list, each pair
+ * masked at each model's own cutoff, their energies and forces added; the list's cutoff is the
+ * larger of the two. */
 typedef struct ndn_potential {
     int kind;
     int num_types;
@@ -122,6 +128,7 @@ typedef struct ndn_potential {
     const void* spacing;
     int64_t knots;
     double cutoff;
+    const struct ndn_potential* next;
 } ndn_potential;
 
 /* Workspace bytes of ndn_pair_efw for a list of `columns` columns. */

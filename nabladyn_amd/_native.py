@@ -45,7 +45,7 @@ class Potential(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int), ("num_types", ctypes.c_int), ("atom_types", _vp),
         ("sigma", _vp), ("epsilon", _vp), ("table", _vp), ("start", _vp), ("spacing", _vp),
-        ("knots", _i64), ("cutoff", ctypes.c_double),
+        ("knots", _i64), ("cutoff", ctypes.c_double), ("next", _vp),
     ]
 
 

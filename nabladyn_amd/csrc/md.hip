@@ -330,6 +330,52 @@ struct OneTypeTable
     }
 };
 
+// One model of a sum of two (potential.sum): LJ or a table, typed, with its own cutoff. The
+// kind is the same for every pair of a launch, so the branch does not diverge.
+template <typename T>
+struct SumPart
+{
+    int kind;
+    TypedLJ<T> lj;
+    TypedTable<T> table;
+    T cutoff_squared;
+
+    __device__ __forceinline__ void eval(T type_i, T type_j, T r2, T& u, T& fs) const
+    {
+        if (kind == NDN_LJ)
+            lj.eval(type_i, type_j, r2, u, fs);
+        else
+            table.eval(type_i, type_j, r2, u, fs);
+        // beyond this model's cutoff the pair counts nothing for it
+        // (potential/_sum/_potential.py:86-99 hands each model the list of its own cutoff)
+        const bool inside = r2 < cutoff_squared;
+        u = inside ? u : T(0);
+        fs = inside ? fs : T(0);
+    }
+};
+
+// A sum of two models in one walk: the pair loop's cutoff is the larger one, each model masks
+// its own; u and fs of the two are added per pair.
+template <typename T>
+struct PairSum
+{
+    template <typename E>
+    static constexpr int waves = sizeof(E) == 4 ? 4 : 1;
+
+    SumPart<T> first, second;
+
+    __device__ __forceinline__ const PairSum& bind() const { return *this; }
+
+    __device__ __forceinline__ void eval(T type_i, T type_j, T r2, T& u, T& fs) const
+    {
+        T u1, f1, u2, f2;
+        first.eval(type_i, type_j, r2, u1, f1);
+        second.eval(type_i, type_j, r2, u2, f2);
+        u = u1 + u2;
+        fs = f1 + f2;
+    }
+};
+
 // LDS a potential's evaluator stages per block (after the halo and the brick's bookkeeping), its
 // staging, and its bound evaluator: nothing for the potentials read from registers or memory.
 template <class Pot>
@@ -545,8 +591,21 @@ __device__ __forceinline__ void box_first_half(const StepArgs<T>& a)
 // ---- prologue -------------------------------------------------------------------------------
 
 template <typename T>
+__device__ __forceinline__ void write_coefficients(const ndn_potential& pot, void* coefficients);
+
+// The prologue's `second` when the model is not a sum.
+__host__ __device__ inline ndn_potential no_second_model()
+{
+    ndn_potential none{};
+    none.kind = -1;
+    return none;
+}
+
+// `second` is the second model of a sum (kind -1 without one), its table at second_table.
+template <typename T>
 __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_potential& pot,
-                                              void* coefficients)
+                                              void* coefficients, const ndn_potential& second,
+                                              void* second_table)
 {
     if (threadIdx.x == 0)
     {
@@ -575,6 +634,15 @@ __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_po
             prune->refresh = 0;
         }
     }
+    write_coefficients<T>(pot, coefficients);
+    if (second.kind >= 0) write_coefficients<T>(second, second_table);
+}
+
+// A model's coefficient table, by the block: LJ rows (sigma^2, 4 eps, 24 eps, the shift at the
+// cutoff), or the table's padded knots [P, K + 2] and spacing [P].
+template <typename T>
+__device__ __forceinline__ void write_coefficients(const ndn_potential& pot, void* coefficients)
+{
     const int rows = pot.num_types * (pot.num_types + 1) / 2;
     if (pot.kind == NDN_LJ)
     {
@@ -620,9 +688,11 @@ __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_po
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) prologue_kernel(StepArgs<T> a, const ndn_potential pot,
-                                                          void* coefficients)
+                                                          void* coefficients,
+                                                          const ndn_potential second,
+                                                          void* second_table)
 {
-    prologue_body(a, pot, coefficients);
+    prologue_body(a, pot, coefficients, second, second_table);
 }
 
 // ---- first half -----------------------------------------------------------------------------
@@ -2464,6 +2534,69 @@ inline unsigned force_grid(std::int64_t bricks)
     return xcd_grid(std::max<std::int64_t>(bricks, 1));
 }
 
+// The larger cutoff of a sum of two, else the model's.
+inline double model_cutoff(const ndn_potential* pot)
+{
+    return pot->next != nullptr ? std::max(pot->cutoff, pot->next->cutoff) : pot->cutoff;
+}
+
+// Coefficient bytes of the MD step's model: a sum's second table follows the first's.
+inline std::int64_t model_coefficient_bytes(const ndn_potential* pot, int element)
+{
+    ndn_potential first = *pot;
+    first.next = nullptr;
+    const std::int64_t bytes = coefficient_bytes(&first, element);
+    return pot->next != nullptr ? bytes + coefficient_bytes(pot->next, element) : bytes;
+}
+
+// Where a sum's second coefficient table starts in dtype T (null without a second model).
+template <typename T>
+void* second_coefficients(const ndn_potential* pot, void* coefficients)
+{
+    if (pot->next == nullptr) return nullptr;
+    ndn_potential first = *pot;
+    first.next = nullptr;
+    return static_cast<unsigned char*>(coefficients) +
+           coefficient_bytes(&first, static_cast<int>(sizeof(T)));
+}
+
+// One model of a sum over its coefficient table (written by the prologue or prepare_model).
+template <typename T>
+SumPart<T> sum_part(const ndn_potential& pot, void* coefficients)
+{
+    SumPart<T> part{};
+    part.kind = pot.kind;
+    part.cutoff_squared = static_cast<T>(pot.cutoff * pot.cutoff);
+    if (pot.kind == NDN_LJ)
+    {
+        part.lj = TypedLJ<T>{static_cast<const LJRow<T>*>(coefficients), pot.num_types};
+    }
+    else
+    {
+        const int rows = pot.num_types * (pot.num_types + 1) / 2;
+        T* padded = static_cast<T*>(coefficients);
+        part.table = TypedTable<T>{Table<T>{
+            padded, static_cast<const T*>(pot.start),
+            padded + static_cast<std::int64_t>(rows) * (pot.knots + 2), nullptr, pot.num_types,
+            static_cast<int>(pot.knots)}};
+    }
+    return part;
+}
+
+// The coefficient tables of the MD step's model from its parameters, outside a step (the
+// prologue writes them in a step).
+template <typename T>
+int prepare_model(const ndn_potential* pot, void* coefficients, hipStream_t stream)
+{
+    ndn_potential first = *pot;
+    first.next = nullptr;
+    PotentialView<T> view;
+    int prepared = prepare_potential<T>(&first, coefficients, stream, &view);
+    if (prepared != NDN_OK || pot->next == nullptr) return prepared;
+    return prepare_potential<T>(pot->next, second_coefficients<T>(pot, coefficients), stream,
+                              &view);
+}
+
 template <typename T>
 OneTypeTable<T> bind_one_type_table(const ndn_potential* pot, void* coefficients)
 {
@@ -2496,11 +2629,19 @@ int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coeffici
     NDN_REQUIRE(a.halo_capacity >= 1 && halo_bytes<T>(a.halo_capacity) <= max_halo_bytes(),
                 "the halo capacity must be at least 1 and its positions fit in the LDS a force "
                 "block may take (ndn_md_halo_limit)");
-    const T cutoff = static_cast<T>(pot->cutoff);
-    const T cutoff_squared = static_cast<T>(pot->cutoff * pot->cutoff);
+    const double reach = model_cutoff(pot);
+    const T cutoff = static_cast<T>(reach);
+    const T cutoff_squared = static_cast<T>(reach * reach);
     const T half_timestep = static_cast<T>(0.5 * a.timestep);
     const int rows = pot->num_types * (pot->num_types + 1) / 2;
-    if (pot->kind == NDN_LJ && pot->num_types == 1)
+    if (pot->next != nullptr)
+    {
+        const PairSum<T> sum{sum_part<T>(*pot, coefficients),
+                             sum_part<T>(*pot->next, second_coefficients<T>(pot, coefficients))};
+        launch_force<T>(a, sum, group, cutoff, cutoff_squared, half_timestep, virial, kick,
+                        partials, grid, stream);
+    }
+    else if (pot->kind == NDN_LJ && pot->num_types == 1)
     {
         const OneTypeLJ<T> lj{static_cast<const LJRow<T>*>(coefficients)};
         launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
@@ -2553,7 +2694,9 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     StepArgs<T> a = step_args<T>(s, settings, list, m);
     const T timestep = static_cast<T>(settings->timestep);
     const T half_timestep = static_cast<T>(0.5 * settings->timestep);
-    prologue_kernel<T><<<1, kBlock, 0, stream>>>(a, *pot, s.coefficients);
+    const ndn_potential second = pot->next != nullptr ? *pot->next : no_second_model();
+    prologue_kernel<T><<<1, kBlock, 0, stream>>>(a, *pot, s.coefficients, second,
+                                                 second_coefficients<T>(pot, s.coefficients));
     NDN_LAUNCHED("md_prologue");
     first_half_kernel<T><<<blocks_for(s.num_atoms), kBlock, 0, stream>>>(a, timestep, half_timestep);
     NDN_LAUNCHED("md_first_half");
@@ -2576,8 +2719,7 @@ int md_evaluate_impl(const ndn_md_system& s, const ndn_potential* pot, const ndn
 {
     StepArgs<T> a = step_args<T>(s, nullptr, nullptr, m);
     a.timestep = timestep;
-    PotentialView<T> view;
-    const int prepared = prepare_potential<T>(pot, s.coefficients, stream, &view);
+    int prepared = prepare_model<T>(pot, s.coefficients, stream);
     if (prepared != NDN_OK) return prepared;
     if (refresh)
     {
@@ -2788,7 +2930,7 @@ __global__ void __launch_bounds__(kBlock) batch_prologue_kernel(BatchTable<T> t)
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
     const ndn_potential pot = m.pot;
-    prologue_body(a, pot, m.coefficients);
+    prologue_body(a, pot, m.coefficients, no_second_model(), nullptr);
 }
 
 template <typename T>
@@ -3154,15 +3296,37 @@ int batch_step_impl(const ndn_md_batch& b, hipStream_t stream)
     return NDN_OK;
 }
 
+// The MD entry points' model: one model, or a sum of two whose parts share the atom types;
+// `element` bytes per value, checked against the system's coefficient buffer.
+int check_md_potential(const ndn_md_system* system, const ndn_potential* pot, int element)
+{
+    NDN_REQUIRE(pot != nullptr, "the potential must not be null");
+    ndn_potential first = *pot;
+    first.next = nullptr;
+    int checked = check_potential(&first, system->num_atoms);
+    if (checked != NDN_OK) return checked;
+    if (pot->next != nullptr)
+    {
+        checked = check_potential(pot->next, system->num_atoms);
+        if (checked != NDN_OK) return checked;
+        NDN_REQUIRE(pot->next->num_types == pot->num_types &&
+                        pot->next->atom_types == pot->atom_types,
+                    "the two models of a sum share num_types and atom_types");
+    }
+    NDN_REQUIRE(system->coefficient_bytes >= model_coefficient_bytes(pot, element),
+                "the coefficient buffer is smaller than ndn_md_workspace asks");
+    return NDN_OK;
+}
+
 // What ndn_md_step checks of one system.
 int check_step(int dtype, const ndn_md_system* system, const ndn_md_settings* settings,
                const ndn_potential* pot, const ndn_list_state* list, const ndn_md_list* md_list)
 {
     int checked = check_system(system, md_list);
     if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     checked = check_potential(pot, system->num_atoms);
     if (checked != NDN_OK) return checked;
-    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     NDN_REQUIRE(settings && list, "settings and list must not be null");
     NDN_REQUIRE(settings->timestep > 0, "the timestep must be positive");
     NDN_REQUIRE(!settings->thermostat || system->chain_masses, "NVT needs chain masses");
@@ -3188,14 +3352,22 @@ extern "C" {
 int ndn_md_workspace(int dtype, const ndn_potential* pot, int64_t num_atoms,
                      int64_t* partial_slots, int64_t* coefficient_bytes_out)
 {
-    const int checked = check_potential(pot, num_atoms);
+    NDN_REQUIRE(pot != nullptr, "the potential must not be null");
+    ndn_potential first = *pot;
+    first.next = nullptr;
+    int checked = check_potential(&first, num_atoms);
     if (checked != NDN_OK) return checked;
+    if (pot->next != nullptr)
+    {
+        checked = check_potential(pot->next, num_atoms);
+        if (checked != NDN_OK) return checked;
+    }
     NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
     NDN_REQUIRE(num_atoms >= 1 && partial_slots && coefficient_bytes_out, "bad arguments");
     // at most one brick per atom holds atoms; the list may ask for fewer (one per brick,
     // rounded up to a multiple of 8)
     *partial_slots = std::max<std::int64_t>(xcd_grid(num_atoms), 8);
-    *coefficient_bytes_out = coefficient_bytes(pot, dtype == NDN_F32 ? 4 : 8);
+    *coefficient_bytes_out = model_coefficient_bytes(pot, dtype == NDN_F32 ? 4 : 8);
     return NDN_OK;
 }
 
@@ -3234,12 +3406,14 @@ int ndn_md_step(int dtype, const ndn_md_system* system, const ndn_md_settings* s
 {
     int checked = check_system(system, md_list);
     if (checked != NDN_OK) return checked;
-    checked = check_potential(pot, system->num_atoms);
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    checked = check_md_potential(system, pot, dtype == NDN_F32 ? 4 : 8);
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(settings && list, "settings and list must not be null");
     NDN_REQUIRE(settings->timestep > 0, "the timestep must be positive");
     NDN_REQUIRE(!settings->thermostat || system->chain_masses, "NVT needs chain masses");
-    NDN_REQUIRE(list->cutoff == pot->cutoff, "the list's cutoff must be the potential's");
+    NDN_REQUIRE(list->cutoff == model_cutoff(pot),
+                "the list's cutoff must be the potential's (a sum's larger one)");
     NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
     NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
                 "several atom types need the slot-ordered types");
@@ -3264,7 +3438,8 @@ int ndn_md_evaluate(int dtype, const ndn_md_system* system, const ndn_potential*
 {
     int checked = check_system(system, md_list);
     if (checked != NDN_OK) return checked;
-    checked = check_potential(pot, system->num_atoms);
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    checked = check_md_potential(system, pot, dtype == NDN_F32 ? 4 : 8);
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
     NDN_REQUIRE(pot->num_types == 1 || md_list->types != nullptr,
@@ -3280,7 +3455,8 @@ int ndn_md_refresh(int dtype, const ndn_md_system* system, const ndn_potential* 
 {
     int checked = check_system(system, md_list);
     if (checked != NDN_OK) return checked;
-    checked = check_potential(pot, system->num_atoms);
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    checked = check_md_potential(system, pot, dtype == NDN_F32 ? 4 : 8);
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
     NDN_REQUIRE(md_list->inner_index && md_list->inner_counts && md_list->prune_positions,
@@ -3318,7 +3494,8 @@ int ndn_md_force(int dtype, const ndn_md_system* system, const ndn_potential* po
 {
     int checked = check_system(system, md_list);
     if (checked != NDN_OK) return checked;
-    checked = check_potential(pot, system->num_atoms);
+    NDN_REQUIRE(dtype == NDN_F32 || dtype == NDN_F64, "dtype must be NDN_F32 or NDN_F64");
+    checked = check_md_potential(system, pot, dtype == NDN_F32 ? 4 : 8);
     if (checked != NDN_OK) return checked;
     NDN_REQUIRE(md_list->stride > 0, "the list has no rows yet");
     if (dtype == NDN_F32)

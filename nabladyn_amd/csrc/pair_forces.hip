@@ -317,6 +317,9 @@ __global__ void __launch_bounds__(kBlock) finish_kernel(const double* __restrict
 int check_potential(const ndn_potential* pot, std::int64_t num_atoms)
 {
     NDN_REQUIRE(pot != nullptr, "the potential must not be null");
+    NDN_REQUIRE(pot->next == nullptr,
+                "a sum of potentials (next) is taken only by ndn_md_workspace, ndn_md_step, "
+                "ndn_md_evaluate, ndn_md_refresh and ndn_md_force");
     NDN_REQUIRE(pot->num_types >= 1, "num_types must be at least 1");
     NDN_REQUIRE(pot->num_types == 1 || pot->atom_types != nullptr,
                 "atom_types is required with more than one type");

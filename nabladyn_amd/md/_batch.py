@@ -79,6 +79,9 @@ class SimulationBatch:
                 f"virial setting; got {sorted(map(str, keys))}")
         for simulation in simulations:
             simulation._require()
+            if simulation._pot.next:
+                raise ValueError("a batch steps simulations of one LennardJones or Tabulated "
+                                 "model each; a sum of two is stepped alone")
         first = simulations[0]
         self.simulations = simulations
         self._device = first._device

@@ -441,8 +441,8 @@ class Simulation:
             raise TypeError("model must be a Potential")
         if not model._fused:
             raise TypeError(
-                "the MD step evaluates a LennardJones or Tabulated model with the library's "
-                "kernels; a sum of models or another energy is not supported here")
+                "the MD step evaluates a LennardJones or Tabulated model, or a sum of two, with "
+                "the library's kernels; another energy is not supported here")
         num_atoms, device, dtype = model.num_atoms, model.device, model.dtype
         _check_frame(model, positions, velocities, masses, cell)
         if images is None:
@@ -555,7 +555,10 @@ class Simulation:
                                        cell, types, inner_skin=inner_skin,
                                        reserve=potential_reserve(model), **grid)
         if types is not None:
+            # the slot-ordered types the force kernel reads, for both models of a sum
             pot.atom_types = _native.ptr(self._neighbors.buffers["types"])
+            if pot.next:
+                self._keep[2].atom_types = pot.atom_types
         slots, coef = _native.ctypes.c_int64(), _native.ctypes.c_int64()
         _native.call("md_workspace", _native.dtype_tag(dtype), _native.ctypes.byref(pot),
                      _native.ctypes.c_int64(num_atoms), _native.ctypes.byref(slots),

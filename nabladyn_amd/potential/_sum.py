@@ -85,6 +85,37 @@ class Sum(Potential):
     def _fused(self) -> bool:
         return all(part._fused for part in self.parts)
 
+    def _native(self):
+        """The C-ABI description the MD step takes for a sum of two LJ or tabulated models that
+        share their atom types: the first model's ``ndn_potential`` with ``next`` at the
+        second's. Any other sum raises ``TypeError``."""
+        import ctypes
+
+        from ._pair import _PairModel
+
+        first, second = (self.parts[0], self.parts[1]) if len(self.parts) == 2 else (None, None)
+        if not (isinstance(first, _PairModel) and isinstance(second, _PairModel)):
+            raise TypeError("the MD step evaluates a sum of exactly two LennardJones or "
+                            "Tabulated models")
+        if first._num_types != second._num_types or not torch.equal(first.atom_types,
+                                                                    second.atom_types):
+            raise TypeError("the MD step evaluates a sum of two models with the same atom types")
+        head, keep_head = first._native()
+        tail, keep_tail = second._native()
+        # equal types: the first model's array serves both
+        tail.atom_types = head.atom_types
+        head.next = ctypes.cast(ctypes.pointer(tail), ctypes.c_void_p)
+        return head, (keep_head, keep_tail, tail)
+
+    @property
+    def atom_types(self) -> Tensor:
+        """The parts' atom types (those of the first)."""
+        return self.parts[0].atom_types
+
+    @property
+    def _num_types(self) -> int:
+        return getattr(self.parts[0], "_num_types", 1)
+
     def _trained_masks(self) -> dict[str, Tensor]:
         masks = {}
         for k, part in enumerate(self.parts):

@@ -10,12 +10,16 @@ Reference files followed:
   exact T over 3N - 3);
 - ``md/_simulation.py:137-196,316-364`` first frame (wrap, COM removal, first force) and the
   step order first_half -> neighbor update -> force + second_half -> closing;
-- ``observables.py:18-41`` K, T, P.
+- ``observables.py:18-41`` K, T, P;
+- ``potential/_sum/_potential.py:86-99`` a sum of models (``SumModel``), each on the pairs within
+  its own cutoff; ``TableModel`` evaluates ``oracle.potentials.t_table_energy``.
 
 Parity status: pinned by invariants. Energy conservation of NVE (drift), zero COM momentum,
 ``maxwell_boltzmann`` giving exactly T, the conserved energy of NVT, and the chain's
 time-reversibility (a half step and its reverse return the start to rounding) are checked in
-tests/test_oracle_md.py. No reference trajectory exists to compare with.
+tests/test_oracle_md.py. No reference trajectory exists to compare with. ``SumModel`` on one
+list equals the parts evaluated each on the list of its own cutoff, as the reference hands them
+out (tests/test_oracle_md.py).
 """
 
 import math
@@ -116,6 +120,38 @@ class LJModel:
         return op.t_lj_energy(
             positions, strain, cell, index, self.atom_types, self.sigma, self.epsilon, self.cutoff
         )
+
+
+class TableModel:
+    """The force evaluation of one tabulated model in fp32/fp64 eager torch."""
+
+    def __init__(self, atom_types, table, start, cutoff, dtype):
+        self.atom_types = torch.as_tensor(atom_types, dtype=torch.int64)
+        self.table = torch.as_tensor(table, dtype=dtype)
+        self.start = torch.as_tensor(start, dtype=dtype)
+        self.cutoff = float(cutoff)
+        # the spacing as the reference computes it, in the table's dtype
+        self.spacing = (self.cutoff - self.start) / (self.table.shape[1] - 1)
+
+    def energy(self, positions, strain, cell, index):
+        return op.t_table_energy(positions, strain, cell, index, self.atom_types, self.table,
+                                 self.start, self.spacing, self.cutoff)
+
+
+class SumModel:
+    """A sum of models (``potential/_sum/_potential.py:86-99``): the parts' energies added in
+    part order, each part masking pairs beyond its own cutoff; the cutoff is the largest."""
+
+    def __init__(self, *parts):
+        self.parts = parts
+        self.cutoff = max(part.cutoff for part in parts)
+
+    def energy(self, positions, strain, cell, index):
+        total = None
+        for part in self.parts:
+            e = part.energy(positions, strain, cell, index)
+            total = e if total is None else total + e
+        return total
 
 
 class CPUList:

@@ -90,3 +90,25 @@ def test_noalloc_entry_point_refuses_what_its_scratch_cannot_hold():
         _native.call("pair_efw_noalloc", _native.F64, ctypes.c_void_p(8), ctypes.c_void_p(8),
                      ctypes.c_int64(1000), ctypes.c_void_p(8), ctypes.c_int64(5000),
                      ctypes.byref(pot), None, None, None, None)
+
+
+def test_pair_entry_points_refuse_a_sum():
+    """``next`` (a sum of two models) is for the MD entry points; the pair entry points refuse
+    it with NDN_ERR_ARGUMENT before anything runs, and ndn_md_workspace sizes both tables."""
+    second = _native.Potential(kind=_native.LJ, num_types=1, sigma=ctypes.c_void_p(8),
+                               epsilon=ctypes.c_void_p(8), cutoff=0.8)
+    pot = _native.Potential(kind=_native.TABLE, num_types=1, table=ctypes.c_void_p(8),
+                            start=ctypes.c_void_p(8), spacing=ctypes.c_void_p(8), knots=200,
+                            cutoff=1.0, next=ctypes.cast(ctypes.pointer(second), ctypes.c_void_p))
+    size = ctypes.c_int64()
+    with pytest.raises(_native.NativeError, match="sum of potentials"):
+        _native.call("pair_workspace_bytes", _native.F32, ctypes.byref(pot), ctypes.c_int64(64),
+                     ctypes.byref(size))
+    slots, alone, both = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _native.call("md_workspace", _native.F32, ctypes.byref(pot), ctypes.c_int64(100),
+                 ctypes.byref(slots), ctypes.byref(both))
+    pot.next = None
+    _native.call("md_workspace", _native.F32, ctypes.byref(pot), ctypes.c_int64(100),
+                 ctypes.byref(slots), ctypes.byref(alone))
+    # the table's 202 padded knots and spacing (fp32, 256-byte multiples), then an LJ row
+    assert alone.value == 1024 and both.value == 1024 + 256

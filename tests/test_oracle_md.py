@@ -64,3 +64,41 @@ def test_nvt_conserved_energy_and_momentum():
         values.append(sim.conserved_energy())
     assert np.abs(np.array(values) - c0).max() / 216 < 3e-4
     assert float((m[:, None] * sim.v).sum(dim=0).abs().max()) < 1e-10
+
+
+def test_sum_model_on_one_list_equals_each_part_on_its_own_list():
+    """The reference hands each model of a sum the list of its own cutoff
+    (potential/_sum/_potential.py:34-43,86-99); the oracle's SumModel masks each part at its
+    cutoff on the longest list. Both give the same energy and forces in fp64 (two types, LJ at
+    0.8 nm plus a table at 1.0 nm); a part left unmasked would not."""
+    rng = np.random.default_rng(12)
+    cell = np.full(3, 2.5)
+    x = on.wrap(on.jittered_lattice(7, 2.5, 0.03, seed=12), cell)[0]
+    types = rng.integers(0, 2, len(x))
+    r = 0.3 + 0.7 / 49 * np.arange(50)
+    table = np.stack([4.0 * ((0.33 / r) ** 12 - (0.33 / r) ** 6) + rng.normal(0, 0.05, 50)
+                      for _ in range(3)]).clip(None, 40.0)
+    table[:, -1] = 0.0
+    lj = omd.LJModel(types, [0.30, 0.32, 0.34], [0.8, 1.0, 1.2], 0.8, torch.float64)
+    tab = omd.TableModel(types, table, [0.3, 0.3, 0.3], 1.0, torch.float64)
+    total = omd.SumModel(lj, tab)
+    assert total.cutoff == 1.0
+    xt, ct = torch.tensor(x), torch.tensor(cell)
+    strain = torch.zeros((3, 3), dtype=torch.float64)
+    far = torch.tensor(on.brute_pairs(x, cell, 1.0).T).long()
+    near = torch.tensor(on.brute_pairs(x, cell, 0.8).T).long()
+
+    def efw(energy, index):
+        g = torch.func.grad_and_value(energy, argnums=0)
+        gx, e = g(xt, strain, ct, index)
+        return float(e), gx
+
+    e_sum, g_sum = efw(total.energy, far)
+    e_lj, g_lj = efw(lj.energy, near)
+    e_tab, g_tab = efw(tab.energy, far)
+    assert abs(e_sum - (e_lj + e_tab)) <= 1e-12 * abs(e_sum)
+    assert float((g_sum - (g_lj + g_tab)).abs().max()) <= 1e-12 * float(g_sum.abs().max())
+    # unmasked, the LJ part would count the pairs between 0.8 and 1.0 nm as well
+    unmasked = omd.LJModel(types, [0.30, 0.32, 0.34], [0.8, 1.0, 1.2], 1.0, torch.float64)
+    e_wide, _ = efw(unmasked.energy, far)
+    assert abs(e_wide - e_lj) > 1e-3 * abs(e_lj)
