@@ -509,6 +509,80 @@ def cpu_baseline(x, cell, steps_per_rebuild):
     }
 
 
+def cpu_host():
+    """The host the CPU figures ran on: logical CPUs, the CPU model, torch's threads."""
+    import torch
+
+    model = None
+    try:
+        with open("/proc/cpuinfo") as info:
+            model = next((line.split(":", 1)[1].strip() for line in info
+                          if line.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "torch_threads": torch.get_num_threads()}
+
+
+def cpu_small_configs(nve_steps=200, systems=16):
+    """The planned CPU figures beside the 2M one, on bounded samples: the 512-atom argon LJ
+    frame (jittered 8^3 lattice, L = 2.90 nm, fp32, skin 0.1 nm) stepped NVE ``nve_steps`` times
+    by the oracle's eager integrator (atom-steps/s), and the per-system loop of E/F/W over the
+    first ``systems`` of the 256 ragged systems (the oracle's C cell list at zero skin, then its
+    autograd E/F/W), in systems/s and atom-evaluations/s of that sample."""
+    import numpy as np
+    import torch
+
+    from nabladyn_amd import md, workloads
+    from oracle import md as omd
+    from oracle import neighbors as on
+    from oracle import potentials as op
+
+    dtype = torch.float32
+    x = torch.tensor(on.jittered_lattice(8, 2.90, 0.02, seed=0), dtype=dtype)
+    masses = torch.full((512,), workloads.ARGON_MASS, dtype=dtype)
+    v = md.maxwell_boltzmann(masses, 120.0, 1).to(dtype)
+    model = omd.LJModel(np.zeros(512, np.int64), [workloads.ARGON_SIGMA],
+                        [workloads.ARGON_EPSILON], CUTOFF, dtype)
+    integrator = omd.Integrator(model, x, v, masses, torch.full((3,), 2.90, dtype=dtype),
+                                dt=0.004, skin=0.1)
+    t0 = time.perf_counter()
+    for _ in range(nve_steps):
+        integrator.step()
+    t_nve = time.perf_counter() - t0
+    raw = workloads.ragged_systems(256, seed=2, min_length=2.45)[:systems]
+    table = torch.tensor(workloads.lj_table()[None], dtype=dtype)
+    start, table_cutoff = torch.tensor([0.3]), 1.2
+    spacing = (table_cutoff - start) / (table.shape[1] - 1)
+    atoms = 0
+    t0 = time.perf_counter()
+    for system in raw:
+        xs = system["positions"].astype(np.float32)
+        cell = system["cell"].astype(np.float32)
+        xw, _ = on.wrap(xs, cell)
+        n = xw.shape[0]
+        atoms += n
+        types = torch.zeros(n, dtype=torch.int64)
+        cutoff = CUTOFF if system["kind"] == "lj" else table_cutoff
+        built = on.build(xw, cell, cutoff)
+        index = torch.as_tensor(built.pairs[:, :built.pair_total]).long()
+        if system["kind"] == "lj":
+            args = (types, torch.tensor([workloads.ARGON_SIGMA]),
+                    torch.tensor([workloads.ARGON_EPSILON]), CUTOFF)
+            op.torch_efw(op.t_lj_energy, torch.tensor(xw), torch.tensor(cell), index, *args)
+        else:
+            op.torch_efw(op.t_table_energy, torch.tensor(xw), torch.tensor(cell), index, types,
+                         table, start, spacing, table_cutoff)
+    t_loop = time.perf_counter() - t0
+    return {
+        "nve_512_atom_steps_per_s": 512 * nve_steps / t_nve,
+        "nve_512_sample": f"{nve_steps} fp32 NVE steps of the 512-atom argon frame",
+        "ragged_systems_per_s": systems / t_loop,
+        "ragged_atom_evals_per_s": atoms / t_loop,
+        "ragged_sample": f"E/F/W of the first {systems} of the 256 ragged systems ({atoms} atoms), "
+                         "fp32",
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -707,6 +781,7 @@ def main() -> int:
         del sim
         torch.cuda.empty_cache()
         result["cpu_baseline"] = cpu_baseline(x, cell, steps_per_rebuild)
+        result["cpu_baseline"].update(host=cpu_host(), small_configs=cpu_small_configs())
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

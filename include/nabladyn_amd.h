@@ -407,7 +407,7 @@ int ndn_md_list_entries(const ndn_md_list* md_list, int64_t num_atoms, int prune
  * computes the same bits as ndn_md_step gives it alone. No reference counterpart: the reference
  * steps one system per compiled step (md/_simulation.py:316-364); a batch is that step for
  * every member. */
-#define NDN_MD_BATCH_KERNELS 14
+#define NDN_MD_BATCH_KERNELS 13
 
 /* One member: the same structures ndn_md_step takes. They are read at preparation only. */
 typedef struct ndn_md_member {

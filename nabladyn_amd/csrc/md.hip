@@ -912,8 +912,8 @@ struct Packed
 
 // One thread per atom, in the sub-bin-sorted (unordered within a sub-bin) array: ranks the atom
 // by atom id among its sub-bin's atoms (a handful), so the new order is (sub-bin, atom id)
-// whatever the atomics did, and gathers its whole frame entry into scratch at its new slot;
-// md_unpack_kernel puts the frame home.
+// whatever the atomics did, and gathers its whole frame entry into scratch at its new slot (and
+// the gather copy positions4 there); md_unpack_body puts the frame home.
 template <typename T>
 __device__ __forceinline__ void md_order_body(const StepArgs<T>& a, const Launch launch,
                                               int forced)
@@ -944,6 +944,13 @@ __device__ __forceinline__ void md_order_body(const StepArgs<T>& a, const Launch
         p.type = a.types ? a.types[old] : 0;
         p.id = id;
         reinterpret_cast<Packed<T>*>(a.scratch)[slot] = p;
+        // the gather copy in the new order, so the fill need not wait for the unpack
+        typename Vec4<T>::type p4;
+        p4.x = p.x[0];
+        p4.y = p.x[1];
+        p4.z = p.x[2];
+        p4.w = static_cast<T>(p.type);
+        reinterpret_cast<typename Vec4<T>::type*>(a.positions4)[slot] = p4;
     }
 }
 
@@ -957,7 +964,6 @@ template <typename T>
 __device__ __forceinline__ void md_unpack_body(const StepArgs<T>& a, const Launch launch,
                                                int forced)
 {
-    using V4 = typename Vec4<T>::type;
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
     for (std::int64_t slot = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
          slot < a.num_atoms; slot += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
@@ -976,24 +982,12 @@ __device__ __forceinline__ void md_unpack_body(const StepArgs<T>& a, const Launc
         a.inverse_masses[slot] = p.inverse_mass;
         if (a.types) a.types[slot] = p.type;
         a.ids[slot] = p.id;
-        V4 p4;
-        p4.x = p.x[0];
-        p4.y = p.x[1];
-        p4.z = p.x[2];
-        p4.w = static_cast<T>(p.type);
-        reinterpret_cast<V4*>(a.positions4)[slot] = p4;
         if (slot == 0)
         {
             for (int k = 0; k < 3; ++k) a.reference_cell[k] = a.cell[k];
             if (a.rebuild_count) ++a.rebuild_count[0];
         }
     }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) md_unpack_kernel(StepArgs<T> a, int forced)
-{
-    md_unpack_body(a, own_launch(), forced);
 }
 
 // Distance from coordinate c to the slab [lo, lo + w): 0 inside.
@@ -1633,11 +1627,27 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
     }
 }
 
+// The unpack and the fill in one launch: neither reads what the other writes (the fill reads the
+// gather copy the order wrote, the halo lists and the sub-bin offsets; the unpack writes the
+// frame home), so the first `unpack_blocks` blocks unpack and the rest fill.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_fill_kernel(StepArgs<T> a, T radius,
-                                                         T cull_squared, int forced)
+__device__ __forceinline__ void md_unpack_fill_body(const StepArgs<T>& a, const Launch launch,
+                                                    index_t unpack_blocks, T radius,
+                                                    T cull_squared, int forced)
 {
-    md_fill_body(a, own_launch(), radius, cull_squared, forced);
+    if (launch.block < unpack_blocks)
+        md_unpack_body(a, Launch{launch.block, unpack_blocks}, forced);
+    else
+        md_fill_body(a, Launch{launch.block - unpack_blocks, launch.blocks - unpack_blocks},
+                     radius, cull_squared, forced);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) md_unpack_fill_kernel(StepArgs<T> a,
+                                                                index_t unpack_blocks, T radius,
+                                                                T cull_squared, int forced)
+{
+    md_unpack_fill_body(a, own_launch(), unpack_blocks, radius, cull_squared, forced);
 }
 
 // ---- forces ---------------------------------------------------------------------------------
@@ -2423,11 +2433,11 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     NDN_LAUNCHED("md_scatter_halo");
     md_order_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_order");
-    md_unpack_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
-    NDN_LAUNCHED("md_unpack");
-    md_fill_kernel<T><<<gated_grid(a.bins, kBlock / kWave), kBlock, 0, stream>>>(
-        a, r, cull * cull, forced);
-    NDN_LAUNCHED("md_fill");
+    const unsigned unpack = gated_grid(n);
+    const unsigned fill = gated_grid(a.bins, kBlock / kWave);
+    md_unpack_fill_kernel<T><<<unpack + fill, kBlock, 0, stream>>>(
+        a, static_cast<index_t>(unpack), r, cull * cull, forced);
+    NDN_LAUNCHED("md_unpack_fill");
     return NDN_OK;
 }
 
@@ -2815,7 +2825,8 @@ enum BatchPotential
 };
 
 // The launches of a step: one per part but the scatter and the halo listing, which share one,
-// and the forces, one launch per kind of potential (kLaunchForce + kind).
+// the unpack and the fill, which share one, and the forces, one launch per kind of potential
+// (kLaunchForce + kind).
 enum BatchLaunch
 {
     kLaunchPrologue,
@@ -2824,8 +2835,7 @@ enum BatchLaunch
     kLaunchScan,
     kLaunchScatterHalo,
     kLaunchOrder,
-    kLaunchUnpack,
-    kLaunchFill,
+    kLaunchUnpackFill,
     kLaunchPrune,
     kLaunchForce,
     kLaunchFinish = kLaunchForce + kBatchPotentials,
@@ -2837,7 +2847,7 @@ static_assert(kBatchLaunches == NDN_MD_BATCH_KERNELS, "the header's launch count
 constexpr int kLaunchParts[kBatchLaunches][2] = {
     {kPartPrologue, kBatchParts}, {kPartFirstHalf, kBatchParts}, {kPartAssign, kBatchParts},
     {kPartScan, kBatchParts},     {kPartScatter, kPartHalo},     {kPartOrder, kBatchParts},
-    {kPartUnpack, kBatchParts},   {kPartFill, kBatchParts},      {kPartPrune, kBatchParts},
+    {kPartUnpack, kPartFill},     {kPartPrune, kBatchParts},
     {kPartForce, kBatchParts},    {kPartForce, kBatchParts},     {kPartForce, kBatchParts},
     {kPartForce, kBatchParts},    {kPartFinish, kBatchParts}};
 
@@ -2953,7 +2963,6 @@ __global__ void __launch_bounds__(kBlock) batch_first_half_kernel(BatchTable<T> 
     }
 NDN_BATCH_GATED(md_assign, kPartAssign)
 NDN_BATCH_GATED(md_order, kPartOrder)
-NDN_BATCH_GATED(md_unpack, kPartUnpack)
 #undef NDN_BATCH_GATED
 
 template <typename T>
@@ -2976,12 +2985,15 @@ __global__ void __launch_bounds__(kBlock) batch_scatter_halo_kernel(BatchTable<T
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) batch_fill_kernel(BatchTable<T> t)
+__global__ void __launch_bounds__(kBlock) batch_unpack_fill_kernel(BatchTable<T> t)
 {
-    const BatchBlock b = batch_block(t.map, t.blocks, kPartFill);
+    const BatchBlock b = batch_block(t.map, t.blocks, kPartUnpack);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
-    md_fill_body(a, b.launch, m.radius, m.cull_squared, 0);
+    const index_t unpack = b.launch.blocks;
+    const index_t fill = t.blocks[b.member * kBatchParts + kPartFill];
+    md_unpack_fill_body(a, Launch{b.launch.block, unpack + fill}, unpack, m.radius,
+                        m.cull_squared, 0);
 }
 
 template <typename T>
@@ -3257,12 +3269,9 @@ int batch_step_impl(const ndn_md_batch& b, hipStream_t stream)
     batch_md_order_kernel<T><<<grid(kLaunchOrder), kBlock, 0, stream>>>(
         batch_table<T>(b, kLaunchOrder));
     NDN_LAUNCHED("md_batch_order");
-    batch_md_unpack_kernel<T><<<grid(kLaunchUnpack), kBlock, 0, stream>>>(
-        batch_table<T>(b, kLaunchUnpack));
-    NDN_LAUNCHED("md_batch_unpack");
-    batch_fill_kernel<T><<<grid(kLaunchFill), kBlock, 0, stream>>>(
-        batch_table<T>(b, kLaunchFill));
-    NDN_LAUNCHED("md_batch_fill");
+    batch_unpack_fill_kernel<T><<<grid(kLaunchUnpackFill), kBlock, 0, stream>>>(
+        batch_table<T>(b, kLaunchUnpackFill));
+    NDN_LAUNCHED("md_batch_unpack_fill");
     if (b.grids[kLaunchPrune] > 0)
     {
         const std::size_t lds = static_cast<std::size_t>(b.lds[kLaunchPrune]);

@@ -19,7 +19,10 @@ import torch
 from .. import _native
 from ._simulation import ListState, MDList, MDSettings, MDSystem, Simulation, _count
 
-_KERNELS = 14
+_KERNELS = 13
+# the launches of a step (ndn_md_batch.grids): the forces, one per kind of potential, come eighth
+# to eleventh: one-type LJ, typed LJ, typed table, one-type table; the finish is last
+FORCE_LAUNCHES = {"lj": 8, "typed_lj": 9, "typed_table": 10, "table": 11}
 _MAX_MEMBERS = 256
 
 

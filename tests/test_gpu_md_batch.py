@@ -100,7 +100,8 @@ def test_mixed_potentials_in_one_batch(device):
         s.run(90)
     batched = [make(graphs=False) for make in makers]
     batch = md.SimulationBatch(batched)
-    assert batch.grids[9] > 0 and batch.grids[12] > 0  # one-type LJ and one-type table forces
+    force = md._batch.FORCE_LAUNCHES
+    assert batch.grids[force["lj"]] > 0 and batch.grids[force["table"]] > 0
     batch.advance(90)
     batch.check()
     for k, (a, b) in enumerate(zip(alone, batched)):
@@ -140,7 +141,8 @@ def test_typed_potentials_in_one_batch(device):
         s.run(80)
     batched = [make(graphs=False) for make in makers]
     batch = md.SimulationBatch(batched)
-    assert batch.grids[10] > 0 and batch.grids[11] > 0  # typed LJ, typed table forces
+    force = md._batch.FORCE_LAUNCHES
+    assert batch.grids[force["typed_lj"]] > 0 and batch.grids[force["typed_table"]] > 0
     batch.advance(80)
     batch.check()
     for k, (a, b) in enumerate(zip(alone, batched)):
@@ -159,4 +161,5 @@ def test_batch_refuses_mixed_and_empty(device):
     table = _members("table", torch.float32, device)[0](graphs=False)
     assert md.batch_key(lj) == md.batch_key(table) != md.batch_key(wide)
     batch = md.SimulationBatch([lj])
-    assert len(batch) == 1 and batch.grids[-1] == 1 and batch.grids[12] == 0
+    assert len(batch) == 1 and batch.grids[-1] == 1
+    assert batch.grids[md._batch.FORCE_LAUNCHES["table"]] == 0
