@@ -211,6 +211,18 @@ int ndn_pair_efw_frames(int dtype, const void* positions, const void* cells, int
                         const ndn_potential* pot, void* energy, void* forces, void* virial,
                         void* workspace, int64_t workspace_bytes, ndn_stream_t stream);
 
+/* ndn_pair_efw_frames over full rows instead of the stacks: frame f's rows are its atoms in
+ * order, row r's k-th neighbor index[f * capacity + slices[f * (S + 1) + r / 64] + r % 64 + 64 k]
+ * for k < counts[f * N + r] (S = ceil(N / 64); the layout of ndn_pair_efw_full per frame, as the
+ * fit builds it once from the frames' stacks). A lane per row sums its force in registers: no
+ * atomics. Same results as ndn_pair_efw_frames to rounding. The workspace is
+ * ndn_pair_frames_workspace_bytes with columns = N. */
+int ndn_pair_efw_full_frames(int dtype, const void* positions, const void* cells, int64_t frames,
+                             int64_t num_atoms, const int32_t* counts, const int32_t* slices,
+                             const int32_t* index, int64_t capacity, const ndn_potential* pot,
+                             void* energy, void* forces, void* virial, void* workspace,
+                             int64_t workspace_bytes, ndn_stream_t stream);
+
 /* ndn_pair_parameter_vjp over the F frames of ndn_pair_efw_frames, summed into one out: upstream
  * grad_energy [F], grad_forces [F, N, 3], grad_virial [F, 3, 3] (any may be NULL). Replaces the
  * vmapped nested grad of training/_loss.py:71-125 with training/_fit.py:245-271. The workspace

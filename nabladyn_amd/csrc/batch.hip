@@ -547,6 +547,22 @@ __global__ void __launch_bounds__(kBlock) batch_table_vjp_kernel(BatchArgs<T> a,
         partials[static_cast<std::int64_t>(blockIdx.x) * width + k] = knot_sums[k];
 }
 
+// Folds the block partials [blocks, K + 2] onto their first `folds` rows: fold f sums rows f,
+// f + folds, f + 2 folds, ... in that order (coalesced rows, a thread per padded knot) and writes
+// the sum over row f, which only fold f reads.
+__global__ void __launch_bounds__(kBlock) table_vjp_fold_kernel(double* __restrict__ partials,
+                                                                int blocks, int width)
+{
+    const int folds = gridDim.x;
+    const int f = blockIdx.x;
+    for (int k = threadIdx.x; k < width; k += blockDim.x)
+    {
+        double sum = 0.0;
+        for (int b = f; b < blocks; b += folds) sum += partials[static_cast<std::int64_t>(b) * width + k];
+        partials[static_cast<std::int64_t>(f) * width + k] = sum;
+    }
+}
+
 // Sums the block partials per padded knot in a fixed order, then maps the padded knots back to
 // the table: the virtual first knot is 3 v0 - 3 v1 + v2, the virtual last 3 v_{K-1} - 3 v_{K-2}
 // + v_{K-3} with v_{K-1} held at 0, so the last knot's gradient is 0.
@@ -680,7 +696,12 @@ int table_vjp_impl(const ndn_batch& b, const void* grad_energy, const void* grad
         a, cull_factor, static_cast<const T*>(grad_energy), static_cast<const T*>(grad_forces),
         static_cast<const T*>(grad_virial), partials);
     NDN_LAUNCHED("batch_table_vjp");
-    table_vjp_finish_kernel<T><<<1, kBlock, 0, stream>>>(partials, static_cast<int>(blocks),
+    // two fixed-order passes: the block rows folded onto at most 512 rows, then summed
+    const int width = static_cast<int>(b.knots) + 2;
+    const int folds = static_cast<int>(std::min<std::int64_t>(blocks, 512));
+    table_vjp_fold_kernel<<<folds, kBlock, 0, stream>>>(partials, static_cast<int>(blocks), width);
+    NDN_LAUNCHED("table_vjp_fold");
+    table_vjp_finish_kernel<T><<<1, kBlock, 0, stream>>>(partials, folds,
                                                          static_cast<int>(b.knots),
                                                          static_cast<T*>(grad_table));
     NDN_LAUNCHED("table_vjp_finish");

@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -1556,53 +1557,62 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
             bool next_valid;
             candidate(lane, next_slot, next_local, next_valid);
             V4 next_raw = pos[next_valid ? next_slot : first];
-            for (index_t chunk = 0; chunk < total; chunk += kWave)
-            {
-                const index_t slot = next_slot, local = next_local;
-                const bool valid = next_valid;
-                const V4 q = frame.place(next_raw);
-                if (chunk + kWave < total)
+            // the frame is the sub-bin's: exact (plain differences) or not (the minimum image per
+            // pair), a compile-time choice inside the chunks
+            auto chunks = [&](auto exact) {
+                for (index_t chunk = 0; chunk < total; chunk += kWave)
                 {
-                    candidate(chunk + kWave + lane, next_slot, next_local, next_valid);
-                    next_raw = pos[next_valid ? next_slot : first];
+                    const index_t slot = next_slot, local = next_local;
+                    const bool valid = next_valid;
+                    const V4 q = frame.place(next_raw);
+                    if (chunk + kWave < total)
+                    {
+                        candidate(chunk + kWave + lane, next_slot, next_local, next_valid);
+                        next_raw = pos[next_valid ? next_slot : first];
+                    }
+                    // the look-ahead read stays here: the scheduler would otherwise sink it to its use
+                    __builtin_amdgcn_sched_barrier(0);
+                    for (int k = 0; k < centers; ++k)
+                    {
+                        const index_t center = first + k;
+                        const T cx = read_lane(mine.x, k);
+                        const T cy = read_lane(mine.y, k);
+                        const T cz = read_lane(mine.z, k);
+                        T dx, dy, dz;
+                        if constexpr (!decltype(exact)::value)
+                        {
+                            dx = minimum_image(cx, q.x, lx);
+                            dy = minimum_image(cy, q.y, ly);
+                            dz = minimum_image(cz, q.z, lz);
+                        }
+                        else
+                        {
+                            dx = cx - q.x;
+                            dy = cy - q.y;
+                            dz = cz - q.z;
+                        }
+                        const T xx = dx * dx;
+                        const T yy = dy * dy;
+                        const T zz = dz * dz;
+                        // no short circuit: the test is straight-line code
+                        const bool hit = valid & (slot != center) & ((xx + yy) + zz < radius_squared);
+                        const std::uint64_t mask = ballot(hit);
+                        const index_t count = read_lane(mine_count, k);
+                        if (hit && writing)
+                        {
+                            const index_t position = count + bits_below(mask);
+                            std::uint16_t* row_out =
+                                a.index + static_cast<std::int64_t>(center) * a.stride;
+                            if (position < stride) row_out[position] = static_cast<std::uint16_t>(local);
+                        }
+                        mine_count += lane == k ? static_cast<index_t>(__popcll(mask)) : 0;
+                    }
                 }
-                // the look-ahead read stays here: the scheduler would otherwise sink it to its use
-                __builtin_amdgcn_sched_barrier(0);
-                for (int k = 0; k < centers; ++k)
-                {
-                    const index_t center = first + k;
-                    const T cx = read_lane(mine.x, k);
-                    const T cy = read_lane(mine.y, k);
-                    const T cz = read_lane(mine.z, k);
-                    T dx, dy, dz;
-                    if (wraps)
-                    {
-                        dx = minimum_image(cx, q.x, lx);
-                        dy = minimum_image(cy, q.y, ly);
-                        dz = minimum_image(cz, q.z, lz);
-                    }
-                    else
-                    {
-                        dx = cx - q.x;
-                        dy = cy - q.y;
-                        dz = cz - q.z;
-                    }
-                    const T xx = dx * dx;
-                    const T yy = dy * dy;
-                    const T zz = dz * dz;
-                    const bool hit = valid && slot != center && (xx + yy) + zz < radius_squared;
-                    const std::uint64_t mask = ballot(hit);
-                    const index_t count = read_lane(mine_count, k);
-                    if (hit && writing)
-                    {
-                        const index_t position = count + bits_below(mask);
-                        std::uint16_t* row_out =
-                            a.index + static_cast<std::int64_t>(center) * a.stride;
-                        if (position < stride) row_out[position] = static_cast<std::uint16_t>(local);
-                    }
-                    mine_count += lane == k ? static_cast<index_t>(__popcll(mask)) : 0;
-                }
-            }
+            };
+            if (wraps)
+                chunks(std::false_type{});
+            else
+                chunks(std::true_type{});
             // the centers' counts, the overflow flag, and the sentinel padding of each row
             if (lane < centers)
             {

@@ -212,12 +212,23 @@ __global__ void __launch_bounds__(kBlock) full_rows_kernel(
     T cutoff_squared, Pot pot, T* __restrict__ forces, double* __restrict__ partials)
 {
     __shared__ double shared[16];
+    const std::int64_t slices_total = (static_cast<std::int64_t>(num_atoms) + kWave - 1) / kWave;
+    // frames of one system (blockIdx.y): each frame's arrays follow the previous frame's (rows
+    // [N], counts [N], slices [N / 64 + 1], `capacity` index entries)
+    const std::int64_t frame = blockIdx.y;
+    positions += 3 * frame * num_atoms;
+    cell += 3 * frame;
+    if (rows != nullptr) rows += frame * num_atoms;
+    counts += frame * num_atoms;
+    slices += frame * (slices_total + 1);
+    index += frame * capacity;
+    if (kForces) forces += 3 * frame * num_atoms;
+    partials += 8 * frame * gridDim.x;
     const T lx = cell[0], ly = cell[1], lz = cell[2];
     const int lane = lane_id();
     T energy = 0;
     T w[6] = {0, 0, 0, 0, 0, 0};
     const std::int64_t waves = static_cast<std::int64_t>(gridDim.x) * (blockDim.x / kWave);
-    const std::int64_t slices_total = (static_cast<std::int64_t>(num_atoms) + kWave - 1) / kWave;
     for (std::int64_t slice =
              static_cast<std::int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
          slice < slices_total; slice += waves)
@@ -434,8 +445,9 @@ template <typename T, class Pot>
 void launch_full(const T* x, const T* cell, const index_t* rows, const index_t* counts,
                  const index_t* slices, const index_t* index, std::int64_t capacity,
                  index_t num_atoms, T cutoff_squared, const Pot& pot, T* forces, bool virial,
-                 double* partials, int blocks, hipStream_t stream)
+                 double* partials, int row_blocks, hipStream_t stream, int frames = 1)
 {
+    const dim3 blocks(static_cast<unsigned>(row_blocks), static_cast<unsigned>(frames));
     if (forces && virial)
         full_rows_kernel<T, Pot, true, true><<<blocks, kBlock, 0, stream>>>(
             x, cell, rows, counts, slices, index, capacity, num_atoms, cutoff_squared, pot,
@@ -487,6 +499,40 @@ int pair_efw_full_impl(const void* positions, const void* cell, std::int64_t num
                        view.table, f, virial != nullptr, partials, blocks, stream);
     NDN_LAUNCHED("full_rows");
     return finish<T>(partials, blocks, energy, virial, 1.0, stream);
+}
+
+// Frames of one system over full rows built from their pair stacks: one launch of F x blocks,
+// one finish block per frame.
+template <typename T>
+int pair_efw_full_frames_impl(const void* positions, const void* cells, std::int64_t frames,
+                              std::int64_t num_atoms, const index_t* counts,
+                              const index_t* slices, const index_t* index,
+                              std::int64_t capacity, const ndn_potential* pot, void* energy,
+                              void* forces, void* virial, void* workspace, hipStream_t stream)
+{
+    PotentialView<T> view;
+    const int prepared = prepare_potential<T>(pot, workspace, stream, &view);
+    if (prepared != NDN_OK) return prepared;
+    const int blocks = full_blocks(num_atoms);
+    auto* partials = reinterpret_cast<double*>(static_cast<std::uint8_t*>(workspace) +
+                                               coefficient_bytes(pot, sizeof(T)));
+    T* f = static_cast<T*>(forces);
+    const T* x = static_cast<const T*>(positions);
+    const T* L = static_cast<const T*>(cells);
+    const index_t n = static_cast<index_t>(num_atoms);
+    const int count = static_cast<int>(frames);
+    if (view.kind == NDN_LJ)
+        launch_full<T>(x, L, nullptr, counts, slices, index, capacity, n, view.cutoff_squared,
+                       view.lj, f, virial != nullptr, partials, blocks, stream, count);
+    else
+        launch_full<T>(x, L, nullptr, counts, slices, index, capacity, n, view.cutoff_squared,
+                       view.table, f, virial != nullptr, partials, blocks, stream, count);
+    NDN_LAUNCHED("full_rows_frames");
+    if (energy == nullptr && virial == nullptr) return NDN_OK;
+    finish_kernel<T><<<count, kBlock, 0, stream>>>(partials, blocks, static_cast<T*>(energy),
+                                                   static_cast<T*>(virial), 1.0);
+    NDN_LAUNCHED("finish_frames");
+    return NDN_OK;
 }
 
 template <typename T>
@@ -636,6 +682,30 @@ int ndn_pair_frames_workspace_bytes(int dtype, const ndn_potential* pot, int64_t
              std::max<std::int64_t>(slots, kPartialSlots) * 8 *
                  static_cast<std::int64_t>(sizeof(double));
     return NDN_OK;
+}
+
+int ndn_pair_efw_full_frames(int dtype, const void* positions, const void* cells, int64_t frames,
+                             int64_t num_atoms, const int32_t* counts, const int32_t* slices,
+                             const int32_t* index, int64_t capacity, const ndn_potential* pot,
+                             void* energy, void* forces, void* virial, void* workspace,
+                             int64_t workspace_bytes, ndn_stream_t stream)
+{
+    const int checked = check_potential(pot, num_atoms);
+    if (checked != NDN_OK) return checked;
+    NDN_REQUIRE(positions && cells && counts && slices && index && workspace, "a buffer is null");
+    NDN_REQUIRE(num_atoms >= 1 && num_atoms <= INT32_MAX, "N must satisfy 1 <= N < 2**31");
+    NDN_REQUIRE(capacity >= 1, "each frame's full list must have a positive capacity");
+    std::int64_t needed = 0;
+    const int sized = ndn_pair_frames_workspace_bytes(dtype, pot, frames, num_atoms, &needed);
+    if (sized != NDN_OK) return sized;
+    NDN_REQUIRE(workspace_bytes >= needed, "the workspace is too small");
+    if (dtype == NDN_F32)
+        return pair_efw_full_frames_impl<float>(positions, cells, frames, num_atoms, counts, slices,
+                                                index, capacity, pot, energy, forces, virial,
+                                                workspace, stream);
+    return pair_efw_full_frames_impl<double>(positions, cells, frames, num_atoms, counts, slices,
+                                             index, capacity, pot, energy, forces, virial,
+                                             workspace, stream);
 }
 
 int ndn_pair_efw_frames(int dtype, const void* positions, const void* cells, int64_t frames,

@@ -442,8 +442,11 @@ int parameter_impl(const void* positions, const void* cell, std::int64_t num_ato
     zero_values<double><<<std::min<unsigned>(blocks_for(out_count), 256u), kBlock, 0, stream>>>(
         out, out_count);
     NDN_LAUNCHED("zero_gradient");
-    const dim3 blocks(static_cast<unsigned>(gradient_blocks(capacity)),
-                      static_cast<unsigned>(frames));
+    // each block adds its rows into `out` once at its end: over many frames, fewer blocks per
+    // frame (each striding over more columns) keep those atomics on the same rows few
+    const std::int64_t per_frame = std::max<std::int64_t>(
+        1, std::min<std::int64_t>(gradient_blocks(capacity), 2048 / frames));
+    const dim3 blocks(static_cast<unsigned>(per_frame), static_cast<unsigned>(frames));
     const T* x = static_cast<const T*>(positions);
     const T* L = static_cast<const T*>(cell);
     const index_t n = static_cast<index_t>(num_atoms);

@@ -154,6 +154,29 @@ def evaluate_frames(model, positions: Tensor, cells: Tensor, stacks: Tensor, nat
     return e, f, w
 
 
+def evaluate_frames_full(model, positions: Tensor, cells: Tensor, rows, native):
+    """``evaluate_frames`` over full rows (``rows``: counts [F, N], slices [F, N / 64 + 1] and
+    index [F, capacity], int32, the layout ``training._loss.frame_rows`` builds): a lane per
+    row, no atomics."""
+    x = positions.detach().contiguous()
+    cells = cells.detach().contiguous()
+    frames, atoms = x.shape[0], x.shape[1]
+    dtype, device = x.dtype, x.device
+    workspace, pot, keep = _frames_workspace(model, frames, atoms, device, native)
+    e = torch.empty(frames, dtype=dtype, device=device)
+    f = torch.empty((frames, atoms, 3), dtype=dtype, device=device)
+    w = torch.empty((frames, 3, 3), dtype=dtype, device=device)
+    counts, slices, index = rows
+    _native.call("pair_efw_full_frames", _native.dtype_tag(dtype), _native.ptr(x),
+                 _native.ptr(cells), _native.ctypes.c_int64(frames), _native.ctypes.c_int64(atoms),
+                 _native.ptr(counts), _native.ptr(slices), _native.ptr(index),
+                 _native.ctypes.c_int64(index.shape[1]), _native.ctypes.byref(pot), _native.ptr(e),
+                 _native.ptr(f), _native.ptr(w), _native.ptr(workspace),
+                 _native.ctypes.c_int64(workspace.numel()), _native.stream(device))
+    del keep
+    return e, f, w
+
+
 def parameter_vjp_frames(model, positions: Tensor, cells: Tensor, stacks: Tensor, grad_energy,
                          grad_forces, grad_virial, native) -> Tensor:
     """``parameter_vjp`` summed over the frames of ``evaluate_frames``; upstream gradients

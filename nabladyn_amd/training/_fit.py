@@ -137,12 +137,19 @@ class Report:
 @dataclasses.dataclass
 class _Prepared:
     """Frames on the model's device and dtype, wrapped into their cells, with one pair stack
-    per list cutoff."""
+    per list cutoff and, for the fused models, the full rows built from each stack."""
 
     positions: Tensor
     cells: Tensor
     stacks: tuple[Tensor, ...]
     labels: dict[str, Tensor]
+    rows: tuple[_loss.FrameRows, ...] | None = None
+
+    def chunk(self, frames: Tensor):
+        """The stacks and rows of ``frames``."""
+        stacks = tuple(stack[frames] for stack in self.stacks)
+        rows = None if self.rows is None else tuple(r.select(frames) for r in self.rows)
+        return stacks, rows
 
 
 def _prepare(model: Potential, data: Dataset, channels: tuple[str, ...]) -> _Prepared:
@@ -153,8 +160,11 @@ def _prepare(model: Potential, data: Dataset, channels: tuple[str, ...]) -> _Pre
     positions = torch.stack([space.wrap(positions[f], cells[f])[0]
                              for f in range(positions.shape[0])]).contiguous()
     stacks = tuple(pair_stack(positions, cells, cutoff) for cutoff in model._list_cutoffs)
+    rows = None
+    if model._fused:
+        rows = tuple(_loss.frame_rows(stack, model.num_atoms) for stack in stacks)
     return _Prepared(positions, cells, stacks,
-                     {name: moved(data.labels[name]) for name in channels})
+                     {name: moved(data.labels[name]) for name in channels}, rows)
 
 
 def _chunks(frames: Tensor, size: int):
@@ -178,10 +188,10 @@ def batch_gradient(model: Potential, trained: dict[str, Tensor], prepared: _Prep
         chunk = chunk.to(device)
         coefficients = torch.full((len(chunk),), 1.0 / len(frames), dtype=model.dtype,
                                   device=device)
+        stacks, rows = prepared.chunk(chunk)
         predicted = _loss._predict(model, leaves, prepared.positions[chunk],
-                                   prepared.cells[chunk],
-                                   tuple(stack[chunk] for stack in prepared.stacks), channels,
-                                   wrapped=True)
+                                   prepared.cells[chunk], stacks, channels, wrapped=True,
+                                   rows=rows)
         labels = {name: prepared.labels[name][chunk] for name in channels}
         value, terms = _loss.loss(predicted, labels, weights, channel_scales, coefficients)
         grads = torch.autograd.grad(value, tuple(leaves.values()), allow_unused=True)
@@ -205,10 +215,10 @@ def _errors(model: Potential, prepared: _Prepared, size: int) -> dict[str, float
     with torch.no_grad():
         for chunk in _chunks(frames, size):
             chunk = chunk.to(prepared.positions.device)
+            stacks, rows = prepared.chunk(chunk)
             predicted = _loss._predict(model, {}, prepared.positions[chunk],
-                                       prepared.cells[chunk],
-                                       tuple(stack[chunk] for stack in prepared.stacks),
-                                       channels, wrapped=True)
+                                       prepared.cells[chunk], stacks, channels, wrapped=True,
+                                       rows=rows)
             for name in channels:
                 error = predicted[name] - prepared.labels[name][chunk]
                 squares[name] = squares[name] + error.double().square().sum()

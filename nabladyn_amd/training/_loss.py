@@ -11,6 +11,7 @@ vmaps. Any other model is differentiated twice by autograd, as the reference doe
 """
 
 from collections.abc import Mapping
+from typing import NamedTuple
 
 import torch
 from torch import Tensor, nn
@@ -18,7 +19,7 @@ from torch import Tensor, nn
 from .. import space
 from ..neighbors import NeighborPairs
 from ..potential import Potential, Sum
-from ..potential._energy import evaluate_frames, parameter_vjp_frames
+from ..potential._energy import evaluate_frames, evaluate_frames_full, parameter_vjp_frames
 
 CHANNELS = ("energy", "forces", "virial")
 
@@ -61,22 +62,72 @@ def loss(predicted: Mapping[str, Tensor], labels: Mapping[str, Tensor],
     return total, terms
 
 
+class FrameRows(NamedTuple):
+    """Full rows of frames built from their pair stacks: row r of frame f (its atom r) lists every
+    pair of the stack with r, in both orientations, 64 rows per slice, column-major within a
+    slice: the k-th neighbor of row r at index[f, slices[f, r // 64] + r % 64 + 64 k] for
+    k < counts[f, r]. All int32."""
+
+    counts: Tensor  # [F, N]
+    slices: Tensor  # [F, ceil(N / 64) + 1]
+    index: Tensor   # [F, capacity]
+
+    def select(self, frames: Tensor) -> "FrameRows":
+        return FrameRows(self.counts[frames], self.slices[frames], self.index[frames])
+
+
+def frame_rows(stack: Tensor, num_atoms: int) -> FrameRows:
+    """The full rows of the frames of a pair stack [F, 2, C] (self-pairs are padding). Each row
+    keeps the stack's order: its pairs as first atom, then as second. Built once per prepared set
+    (the frames do not move), so the fused forward sums each atom's force in registers."""
+    frames, _, columns = stack.shape
+    device = stack.device
+    first, second = stack[:, 0].long(), stack[:, 1].long()
+    listed = first != second
+    frame = torch.arange(frames, device=device)[:, None].expand(frames, columns)
+    owner = torch.cat([frame[listed], frame[listed]])
+    row = torch.cat([first[listed], second[listed]])
+    neighbor = torch.cat([second[listed], first[listed]])
+    key = owner * num_atoms + row
+    order = torch.argsort(key, stable=True)
+    key, neighbor = key[order], neighbor[order]
+    counts = torch.bincount(key, minlength=frames * num_atoms)
+    starts = torch.cumsum(counts, 0) - counts
+    rank = torch.arange(key.numel(), device=device) - starts[key]
+    per_frame = -(-num_atoms // 64)
+    padded = torch.zeros(frames, per_frame * 64, dtype=counts.dtype, device=device)
+    padded[:, :num_atoms] = counts.view(frames, num_atoms)
+    width = padded.view(frames, per_frame, 64).amax(dim=2)
+    slices = torch.zeros(frames, per_frame + 1, dtype=torch.int64, device=device)
+    slices[:, 1:] = torch.cumsum(64 * width, dim=1)
+    capacity = max(int(slices[:, -1].max()), 1)
+    index = torch.zeros(frames, capacity, dtype=torch.int32, device=device)
+    f, r = key // num_atoms, key % num_atoms
+    index[f, slices[f, r // 64] + r % 64 + 64 * rank] = neighbor.to(torch.int32)
+    return FrameRows(counts.view(frames, num_atoms).to(torch.int32).contiguous(),
+                     slices.to(torch.int32).contiguous(), index)
+
+
 class _FusedFrames(torch.autograd.Function):
     """(E [B], F [B, N, 3], W [B, 3, 3]) of an LJ or tabulated model on B frames (wrapped
     positions, pair stacks [B, 2, C]), differentiable in the model's two parameter tensors (LJ:
-    sigma and epsilon; a table: the table and None). Forward: ``ndn_pair_efw_frames``, one
-    launch for every frame; backward: ``ndn_pair_parameter_vjp_frames``, one launch. The
-    positions and the cells are data here and get no gradient."""
+    sigma and epsilon; a table: the table and None). Forward: ``ndn_pair_efw_full_frames`` over
+    the frames' full rows when they are given, else ``ndn_pair_efw_frames`` over the stacks, one
+    launch for every frame; backward: ``ndn_pair_parameter_vjp_frames`` over the stacks, one
+    launch. The positions and the cells are data here and get no gradient."""
 
     @staticmethod
-    def forward(positions, cells, stack, model, first, second):
+    def forward(positions, cells, stack, model, first, second, rows):
         native = model._native()
-        e, f, w = evaluate_frames(model, positions, cells, stack, native)
+        if rows is not None:
+            e, f, w = evaluate_frames_full(model, positions, cells, rows, native)
+        else:
+            e, f, w = evaluate_frames(model, positions, cells, stack, native)
         return e, f, w, native
 
     @staticmethod
     def setup_context(ctx, inputs, output):
-        positions, cells, stack, model, first, second = inputs
+        positions, cells, stack, model, first, second, rows = inputs
         ctx.set_materialize_grads(False)
         ctx.model, ctx.native = model, output[3]
         ctx.save_for_backward(positions, cells, stack)
@@ -88,30 +139,32 @@ class _FusedFrames(torch.autograd.Function):
             raise RuntimeError("training channels give no gradient in the positions or the cells")
         wanted = ctx.needs_input_grad[4] or ctx.needs_input_grad[5]
         if not wanted or (grad_energy is None and grad_forces is None and grad_virial is None):
-            return None, None, None, None, None, None
+            return None, None, None, None, None, None, None
         g = parameter_vjp_frames(ctx.model, positions, cells, stack, grad_energy, grad_forces,
                                  grad_virial, ctx.native)
         first, second = ctx.model._split_parameter_gradient(g.to(positions.dtype))
         return (None, None, None, None, first if ctx.needs_input_grad[4] else None,
-                second if ctx.needs_input_grad[5] else None)
+                second if ctx.needs_input_grad[5] else None, None)
 
 
-def _stack_for(stacks: dict[float, Tensor], cutoff: float) -> Tensor:
-    """The stack of the smallest list cutoff that reaches ``cutoff`` (``list_for``'s rule)."""
-    return stacks[min(c for c in stacks if c >= cutoff)]
+def _list_for(lists: dict, cutoff: float):
+    """The entry of the smallest list cutoff that reaches ``cutoff`` (``list_for``'s rule)."""
+    return lists[min(c for c in lists if c >= cutoff)]
 
 
-def _fused_frames(model, wrapped: Tensor, cells: Tensor, stacks: dict[float, Tensor]):
+def _fused_frames(model, wrapped: Tensor, cells: Tensor, stacks: dict[float, Tensor],
+                  rows: dict[float, FrameRows] | None):
     """(E, F, W) of a fused model (LJ, a table, or a sum of fused models) on B frames, each
     part on the shortest list that reaches its cutoff, added in part order."""
     if isinstance(model, Sum):
-        parts = [_fused_frames(part, wrapped, cells, stacks) for part in model.parts]
+        parts = [_fused_frames(part, wrapped, cells, stacks, rows) for part in model.parts]
         return tuple(sum(values[1:], values[0]) for values in zip(*parts))
-    stack = _stack_for(stacks, model.cutoff)
+    stack = _list_for(stacks, model.cutoff)
     if stack.dtype != torch.int32 or not stack.is_contiguous():
         stack = stack.to(torch.int32).contiguous()
+    own_rows = None if rows is None else _list_for(rows, model.cutoff)
     first, second = model._differentiable_parameters()
-    e, f, w, _ = _FusedFrames.apply(wrapped, cells, stack, model, first, second)
+    e, f, w, _ = _FusedFrames.apply(wrapped, cells, stack, model, first, second, own_rows)
     return e, f, w
 
 
@@ -134,7 +187,8 @@ class _Channels(nn.Module):
         self.model = model
         self.wrapped = wrapped
 
-    def forward(self, positions: Tensor, cells: Tensor, stacks: tuple[Tensor, ...]):
+    def forward(self, positions: Tensor, cells: Tensor, stacks: tuple[Tensor, ...],
+                rows: tuple[FrameRows, ...] | None = None):
         model = self.model
         cutoffs = model._list_cutoffs
         if model._fused:
@@ -143,7 +197,8 @@ class _Channels(nn.Module):
                 x = torch.stack([space.wrap(positions[f], cells[f])[0]
                                  for f in range(positions.shape[0])])
             return _fused_frames(model, x.contiguous(), cells.contiguous(),
-                                 dict(zip(cutoffs, stacks)))
+                                 dict(zip(cutoffs, stacks)),
+                                 None if rows is None else dict(zip(cutoffs, rows)))
         out = []
         for frame in range(positions.shape[0]):
             lists = [NeighborPairs(stack[frame], model.num_atoms, cutoff)
@@ -168,12 +223,15 @@ def predict(model: Potential, parameters: Mapping[str, Tensor], positions: Tenso
     return _predict(model, parameters, positions, cells, stacks, channels, wrapped=False)
 
 
-def _predict(model, parameters, positions, cells, stacks, channels, *, wrapped: bool):
+def _predict(model, parameters, positions, cells, stacks, channels, *, wrapped: bool,
+             rows=None):
     """``predict``; ``wrapped`` says the positions already lie in [0, L) (wrapped by the
-    library's wrap, as a prepared set's are), so the fused path skips wrapping them again."""
+    library's wrap, as a prepared set's are), so the fused path skips wrapping them again;
+    ``rows`` (one ``FrameRows`` per stack) lets the fused forward run over full rows."""
     wrapper = _Channels(model, wrapped)
     named = {f"model.{name}": value for name, value in parameters.items()}
-    values = torch.func.functional_call(wrapper, named, (positions, cells, tuple(stacks)))
+    values = torch.func.functional_call(wrapper, named,
+                                        (positions, cells, tuple(stacks), rows))
     return {name: values[CHANNELS.index(name)] for name in channels}
 
 

@@ -299,8 +299,9 @@ def test_generic_model_trains_through_autograd(device):
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-6)])
 def test_frames_channels_match_each_frame_alone(device, dtype, tol):
-    """E, F and W of a batch of frames from one launch (``ndn_pair_efw_frames``) against each
-    frame alone through ``quantity.energy_force_virial`` on its own list."""
+    """E, F and W of a batch of frames from one launch, over the stacks (``ndn_pair_efw_frames``)
+    and over the full rows built from them (``ndn_pair_efw_full_frames``), against each frame
+    alone through ``quantity.energy_force_virial`` on its own list."""
     from nabladyn_amd import quantity
     from nabladyn_amd.neighbors import NeighborList
 
@@ -308,13 +309,17 @@ def test_frames_channels_match_each_frame_alone(device, dtype, tol):
     model = _models("sum", types, dtype, device)
     x, c = xs.to(device), cells.to(device)
     stacks = tuple(training.pair_stack(x, c, cutoff) for cutoff in model._list_cutoffs)
+    rows = tuple(training._loss.frame_rows(stack, ATOMS) for stack in stacks)
     with torch.no_grad():
-        batched = training.predict(model, {}, x, c, stacks)
+        over_stacks = training.predict(model, {}, x, c, stacks)
+        over_rows = training._loss._predict(model, {}, x, c, stacks, training.CHANNELS,
+                                            wrapped=False, rows=rows)
     efw = quantity.energy_force_virial(model)
     for f in range(3):
         nl = NeighborList(1.0, inner_cutoffs=(0.8,))
         nl.allocate(x[f], c[f])
         e, force, w = efw(x[f], c[f], nl.tensors())
-        for got, want in ((batched["energy"][f], e), (batched["forces"][f], force),
-                          (batched["virial"][f], w)):
-            assert _rel(got.double().cpu(), want.double().cpu()) <= tol, f
+        for batched in (over_stacks, over_rows):
+            for got, want in ((batched["energy"][f], e), (batched["forces"][f], force),
+                              (batched["virial"][f], w)):
+                assert _rel(got.double().cpu(), want.double().cpu()) <= tol, f

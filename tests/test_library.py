@@ -112,3 +112,33 @@ def test_pair_entry_points_refuse_a_sum():
                  ctypes.byref(slots), ctypes.byref(alone))
     # the table's 202 padded knots and spacing (fp32, 256-byte multiples), then an LJ row
     assert alone.value == 1024 and both.value == 1024 + 256
+
+
+def test_force_launch_refuses_too_few_partial_slots():
+    """The host guard of the MD force launch: one fp64 partial slot per block (a block per brick,
+    rounded up to a multiple of 8) must fit the system's partials, else NDN_ERR_ARGUMENT before
+    any launch (the pointers below are never read)."""
+    from nabladyn_amd.md._simulation import MDList, MDSystem
+
+    fake = ctypes.c_void_p(256)
+    system = MDSystem(num_atoms=1000, positions=fake, velocities=fake, forces=fake, images=fake,
+                      masses=fake, inverse_masses=fake, cell=fake, energy=fake, virial=fake,
+                      thermostat=fake, chain_masses=fake, kinetic=fake, partials=fake,
+                      partial_slots=8, coefficients=fake, coefficient_bytes=4096)
+    md_list = MDList(group=4, stride=64, scan_temp_bytes=1 << 20, halo_capacity=100, bricks=75,
+                     **{name: fake for name in (
+                         "bin_counts", "bin_offsets", "bin_of", "unordered", "order", "counts",
+                         "index", "ids", "positions4", "scratch", "scan_temp", "status",
+                         "prepare_dims", "prepare_widths", "prepare_total", "halo_slots",
+                         "halo_runs", "halo_counts", "grid_widths")})
+    md_list.dims[:] = (20, 20, 20)
+    md_list.brick[:] = (4, 4, 8)  # 5 x 5 x 3 = 75 bricks: 80 blocks
+    pot = _native.Potential(kind=_native.LJ, num_types=1, sigma=fake, epsilon=fake, cutoff=1.0)
+    with pytest.raises(_native.NativeError, match="too few partial slots"):
+        _native.call("md_force", _native.F32, ctypes.byref(system), ctypes.byref(pot),
+                     ctypes.byref(md_list), 0, 1, ctypes.c_double(0.004), None)
+    system.partial_slots = 80
+    system.coefficient_bytes = 16
+    with pytest.raises(_native.NativeError, match="coefficient buffer"):
+        _native.call("md_force", _native.F32, ctypes.byref(system), ctypes.byref(pot),
+                     ctypes.byref(md_list), 0, 1, ctypes.c_double(0.004), None)

@@ -135,3 +135,31 @@ def test_fit_checks_its_arguments():
     unlabeled = training.Dataset(**{k: v for k, v in _data().items() if k != "virial"})
     with pytest.raises(ValueError, match="no virial labels"):
         training.fit(_lj(), unlabeled, sgd, updates=1, batch_size=1, virial=1.0)
+
+
+def test_frame_rows_list_every_pair_both_ways_in_stack_order():
+    """The full rows built from a stack [F, 2, C]: row r of frame f holds, in the stack's column
+    order, the second atom of each pair with r first, then the first atom of each pair with r
+    second; self-pairs (the padding) are left out; slices are 64 rows wide, column-major."""
+    from nabladyn_amd.training._loss import frame_rows
+
+    rng = np.random.default_rng(3)
+    frames, atoms, columns = 3, 150, 700
+    stack = torch.tensor(rng.integers(0, atoms, (frames, 2, columns)), dtype=torch.int32)
+    stack[:, :, -40:] = torch.arange(40, dtype=torch.int32) % atoms  # padding
+    rows = frame_rows(stack, atoms)
+    per_frame = -(-atoms // 64)
+    assert rows.counts.shape == (frames, atoms) and rows.slices.shape == (frames, per_frame + 1)
+    for f in range(frames):
+        first, second = stack[f, 0].tolist(), stack[f, 1].tolist()
+        for r in range(atoms):
+            want = [b for a, b in zip(first, second) if a == r and a != b]
+            want += [a for a, b in zip(first, second) if b == r and a != b]
+            count = int(rows.counts[f, r])
+            base = int(rows.slices[f, r // 64]) + r % 64
+            got = [int(rows.index[f, base + 64 * k]) for k in range(count)]
+            assert got == want, (f, r)
+        widths = rows.counts[f].tolist() + [0] * (per_frame * 64 - atoms)
+        for s in range(per_frame):
+            assert int(rows.slices[f, s + 1] - rows.slices[f, s]) == 64 * max(
+                widths[64 * s:64 * s + 64])
