@@ -298,9 +298,10 @@ def bench_fitting(torch, repeats=5, cpu_frames=2):
     200-knot table on [0.3, 1.2] nm (last knot 0), targets from LJ(0.47 nm, 3.5 kJ/mol) at
     rc 1.2 nm (forces plus N(0, 5) noise; E and W exact), fp32, loss E + F + W. Times one
     update's batch gradient over all 64 frames (HIP events, mean of ``repeats``) through
-    ``training`` (a zero-skin pair stack per frame; ndn_pair_efw forward and
-    ndn_pair_parameter_vjp backward per frame) and through ``fit.TableFrames`` (the batched
-    engine, all frames in one launch, and ndn_batch_table_vjp), and the oracle's autograd
+    ``training`` (a zero-skin pair stack per frame and full rows built from it once; one
+    ndn_pair_efw_full_frames forward and one ndn_pair_parameter_vjp_frames backward for all
+    the frames) and through ``fit.TableFrames`` (the batched engine, all frames in one launch,
+    and ndn_batch_table_vjp), and the oracle's autograd
     double backward on ``cpu_frames`` frames on the host. Reports the loss in the reference's
     scaled form and as plain MSE, and the two GPU routes' gradient distance."""
     import numpy as np
