@@ -1409,13 +1409,22 @@ __device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const
     return f;
 }
 
+// Centers a fill wave tests together: their coordinates and counts stay in scalar registers.
+template <typename T>
+constexpr int kFillCenters = sizeof(T) == 4 ? 8 : 4;
+
+// How a fill wave forms a displacement: plain differences of raw positions, of positions placed
+// in the sub-bin's frame, or the minimum image per pair.
+constexpr int kFramePlain = 0, kFramePlaced = 1, kFrameImage = 2;
+
 // One wave per sub-bin. Its atoms (the centers) share one stencil: lanes 0..49 hold the slot
 // ranges of the 25 stencil rows (runs of x-consecutive sub-bins; two parts where a run wraps
 // around x), culled by the distance between sub-bins, so every sub-bin a pair within R could
 // touch is kept, and the offset from a slot of the range to its local index in the brick's halo
 // (each range lies inside one run of the halo). The candidates are packed 64 at a time; each
-// chunk is gathered once and tested against every center in turn with the exact minimum-image
-// expression, and each center's hits are ballot-compacted into its row, in the order met.
+// chunk is gathered once and tested against a group of centers in turn with the exact
+// minimum-image expression, and each center's hits are ballot-compacted into its row, in the
+// order met (a sub-bin with more atoms than a group walks its candidates once per group).
 // Rows are then padded with the sentinel 0 up to their count rounded up to 32, so the force kernel
 // reads whole blocks unmasked. With stride 0 it only counts (the first frame, to size the rows).
 template <typename T>
@@ -1527,93 +1536,65 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
         }
         const index_t inclusive = wave_inclusive_sum(length);
         const index_t start = inclusive - length;
-        const index_t total = __shfl(inclusive, kWave - 1, kWave);
-        // without a frame (a short axis) the images come from the minimum image per pair
-        const bool wraps = !frame.exact;
-        // the centers, 64 at a time: lane k holds center first + k and its count
-        for (index_t first = begin; first < end; first += kWave)
+        // wave-uniform (a scalar read), like everything the walk's masks are built from
+        const index_t total = read_lane(inclusive, kWave - 1);
+        // The frame kind: kFramePlain where placing is the identity for every candidate (the
+        // stencil's sub-bins lie inside the box, far from the half-box distance), kFramePlaced
+        // where the sub-bin's frame moves some (plain differences of placed positions),
+        // kFrameImage without a frame (a short axis: the minimum image per pair).
+        int kind = frame.exact ? kFramePlain : kFrameImage;
+        if (kind == kFramePlain)
         {
-            const int centers = end - first < kWave ? end - first : kWave;
-            const V4 mine = frame.place(pos[lane < centers ? first + lane : first]);
-            index_t mine_count = 0;
-            // candidate t of the stencil: its slot, its local index in the brick's halo (past
-            // an overflowing halo's capacity the sentinel; the status reports it), whether t is
-            // one; the range holding t is the last lane whose start is <= t
-            auto candidate = [&](index_t t, index_t& slot, index_t& local, bool& valid) {
-                int r = 0;
+            const index_t at[3] = {hx, hy, hz};
+            for (int axis = 0; axis < 3; ++axis)
+                if (frame.moved[axis] || dims[axis] < 8 || at[axis] < 2 ||
+                    at[axis] + 2 > dims[axis] - 1)
+                    kind = kFramePlaced;
+        }
+        // candidate t of the stencil: its slot, its local index in the brick's halo (past an
+        // overflowing halo's capacity the sentinel; the status reports it), whether t is one;
+        // the range holding t is the last lane whose start is <= t
+        auto candidate = [&](index_t t, index_t& slot, index_t& local, bool& valid) {
+            int r = 0;
 #pragma unroll
-                for (int step = 32; step > 0; step >>= 1)
-                {
-                    const index_t s = __shfl(start, r + step, kWave);
-                    if (s <= t) r += step;
-                }
-                valid = t < total;
-                slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
-                local = slot + __shfl(to_local, r, kWave);
-                local = local <= a.halo_capacity ? local : 0;
-            };
-            // each chunk's positions are read while the previous chunk is tested
-            index_t next_slot, next_local;
-            bool next_valid;
-            candidate(lane, next_slot, next_local, next_valid);
-            V4 next_raw = pos[next_valid ? next_slot : first];
-            // the frame is the sub-bin's: exact (plain differences) or not (the minimum image per
-            // pair), a compile-time choice inside the chunks
-            auto chunks = [&](auto exact) {
-                for (index_t chunk = 0; chunk < total; chunk += kWave)
-                {
-                    const index_t slot = next_slot, local = next_local;
-                    const bool valid = next_valid;
-                    const V4 q = frame.place(next_raw);
-                    if (chunk + kWave < total)
-                    {
-                        candidate(chunk + kWave + lane, next_slot, next_local, next_valid);
-                        next_raw = pos[next_valid ? next_slot : first];
-                    }
-                    // the look-ahead read stays here: the scheduler would otherwise sink it to its use
-                    __builtin_amdgcn_sched_barrier(0);
-                    for (int k = 0; k < centers; ++k)
-                    {
-                        const index_t center = first + k;
-                        const T cx = read_lane(mine.x, k);
-                        const T cy = read_lane(mine.y, k);
-                        const T cz = read_lane(mine.z, k);
-                        T dx, dy, dz;
-                        if constexpr (!decltype(exact)::value)
-                        {
-                            dx = minimum_image(cx, q.x, lx);
-                            dy = minimum_image(cy, q.y, ly);
-                            dz = minimum_image(cz, q.z, lz);
-                        }
-                        else
-                        {
-                            dx = cx - q.x;
-                            dy = cy - q.y;
-                            dz = cz - q.z;
-                        }
-                        const T xx = dx * dx;
-                        const T yy = dy * dy;
-                        const T zz = dz * dz;
-                        // no short circuit: the test is straight-line code
-                        const bool hit = valid & (slot != center) & ((xx + yy) + zz < radius_squared);
-                        const std::uint64_t mask = ballot(hit);
-                        const index_t count = read_lane(mine_count, k);
-                        if (hit && writing)
-                        {
-                            const index_t position = count + bits_below(mask);
-                            std::uint16_t* row_out =
-                                a.index + static_cast<std::int64_t>(center) * a.stride;
-                            if (position < stride) row_out[position] = static_cast<std::uint16_t>(local);
-                        }
-                        mine_count += lane == k ? static_cast<index_t>(__popcll(mask)) : 0;
-                    }
-                }
-            };
-            if (wraps)
-                chunks(std::false_type{});
-            else
-                chunks(std::true_type{});
-            // the centers' counts, the overflow flag, and the sentinel padding of each row
+            for (int step = 32; step > 0; step >>= 1)
+            {
+                const index_t s = __shfl(start, r + step, kWave);
+                if (s <= t) r += step;
+            }
+            valid = t < total;
+            slot = __shfl(lo, r, kWave) + (t - __shfl(start, r, kWave));
+            local = slot + __shfl(to_local, r, kWave);
+            local = local <= a.halo_capacity ? local : 0;
+        };
+        // Every slot is one candidate, and the sub-bin's own slots are consecutive candidates
+        // (they lie in one range): center `first` + k is candidate own_first(first) + k.
+        auto own_first = [&](index_t first) {
+            const std::uint64_t holder =
+                ballot(length > 0 && lo <= first && first < lo + length);
+            if (holder == 0) return total;
+            const int r = __builtin_ctzll(holder);
+            return read_lane(start, r) + (first - read_lane(lo, r));
+        };
+        // Center k's hits in one chunk, `mask` (its own candidate excluded), go to its
+        // row at `count` onwards; a row that overflows keeps its first `stride` entries (the
+        // status reports it).
+        auto write_hits = [&](std::uint64_t mask, index_t count, char* rows, int k, index_t local) {
+            const bool hit = __builtin_amdgcn_inverse_ballot_w64(mask);
+            const unsigned position =
+                static_cast<unsigned>(count) +
+                __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+            // `rows` is the group's first row, wave-uniform; the entry is a 32-bit byte offset
+            // from it (k < 64 rows on, each shorter than a brick's halo with its margin)
+            if (hit && position < static_cast<unsigned>(stride))
+                *reinterpret_cast<std::uint16_t*>(
+                    rows + ((position + static_cast<unsigned>(k * stride)) << 1)) =
+                    static_cast<std::uint16_t>(local);
+        };
+        // the group's counts (lane k holds center first + k's), the overflow flag, and the
+        // sentinel padding of each row
+        auto close_rows = [&](index_t first, int centers, index_t mine_count) {
             if (lane < centers)
             {
                 a.counts[first + lane] = mine_count;
@@ -1627,12 +1608,148 @@ __device__ __forceinline__ void md_fill_body(const StepArgs<T>& a, const Launch 
                 for (int k = 0; k < centers; ++k)
                 {
                     const index_t count = read_lane(mine_count, k);
-                    std::uint16_t* row_out =
-                        a.index + static_cast<std::int64_t>(first + k) * a.stride;
+                    std::uint16_t* row_out = a.index + static_cast<std::int64_t>(first + k) * a.stride;
                     const index_t padded = (count + 31) & ~31;
                     for (index_t p = count + lane; p < padded && p < stride; p += kWave)
                         row_out[p] = 0;
                 }
+        };
+        if (kind != kFrameImage)
+        {
+            // kFillCenters<T> centers at a time: their coordinates and counts are wave-uniform
+            // (scalar registers) for the whole walk over the candidates, so a test costs the
+            // displacement, the compare and, on a hit, the compacted store
+            constexpr int kCenters = kFillCenters<T>;
+            for (index_t first = begin; first < end; first += kCenters)
+            {
+                const int centers = end - first < kCenters ? end - first : kCenters;
+                const V4 mine = frame.place(pos[lane < centers ? first + lane : first]);
+                T cx[kCenters], cy[kCenters], cz[kCenters];
+                index_t count[kCenters];
+#pragma unroll
+                for (int k = 0; k < kCenters; ++k)
+                {
+                    cx[k] = read_lane(mine.x, k);
+                    cy[k] = read_lane(mine.y, k);
+                    cz[k] = read_lane(mine.z, k);
+                    count[k] = 0;
+                }
+                const index_t own = own_first(first);
+                char* rows = reinterpret_cast<char*>(a.index + static_cast<std::int64_t>(first) * a.stride);
+                // each chunk's positions are read while the previous chunk is tested
+                index_t next_slot, next_local;
+                bool next_valid;
+                candidate(lane, next_slot, next_local, next_valid);
+                V4 next_raw = pos[next_valid ? next_slot : first];
+                // whether the frame moves candidates, and whether rows are written or only
+                // counted, are compile-time choices inside the walk
+                auto chunks = [&](auto placed, auto writes) {
+                    for (index_t chunk = 0; chunk < total; chunk += kWave)
+                    {
+                        const index_t local = next_local;
+                        const std::uint64_t valid = ballot(next_valid);
+                        const V4 q = decltype(placed)::value ? frame.place(next_raw) : next_raw;
+                        if (chunk + kWave < total)
+                        {
+                            candidate(chunk + kWave + lane, next_slot, next_local, next_valid);
+                            next_raw = pos[next_valid ? next_slot : first];
+                        }
+                        // the look-ahead read stays here: the scheduler would otherwise sink it
+                        __builtin_amdgcn_sched_barrier(0);
+                        // this lane's candidate is center k's own where rank == k (held
+                        // opaque, so each center's test compares it with the constant k rather
+                        // than keeping a rebased copy per center)
+                        index_t rank = chunk + lane - own;
+                        asm volatile("" : "+v"(rank));
+#pragma unroll
+                        for (int k = 0; k < kCenters; ++k)
+                        {
+                            if (k < centers)
+                            {
+                                const T dx = cx[k] - q.x;
+                                const T dy = cy[k] - q.y;
+                                const T dz = cz[k] - q.z;
+                                const T xx = dx * dx;
+                                const T yy = dy * dy;
+                                const T zz = dz * dz;
+                                // each ballot of one compare is that compare's mask
+                                const std::uint64_t mask =
+                                    ballot((xx + yy) + zz < radius_squared) & ballot(rank != k) &
+                                    valid;
+                                if constexpr (decltype(writes)::value)
+                                    write_hits(mask, count[k], rows, k, local);
+                                count[k] += static_cast<index_t>(__popcll(mask));
+                                // the store and the count stay here: sunk below the later
+                                // centers' tests, they would keep every center's mask live
+                                asm volatile("" : "+s"(count[k]) : : "memory");
+                            }
+                        }
+                    }
+                };
+                if (writing)
+                {
+                    if (kind == kFramePlain)
+                        chunks(std::false_type{}, std::true_type{});
+                    else
+                        chunks(std::true_type{}, std::true_type{});
+                }
+                else
+                {
+                    if (kind == kFramePlain)
+                        chunks(std::false_type{}, std::false_type{});
+                    else
+                        chunks(std::true_type{}, std::false_type{});
+                }
+                index_t mine_count = count[0];
+#pragma unroll
+                for (int k = 1; k < kCenters; ++k) mine_count = lane == k ? count[k] : mine_count;
+                close_rows(first, centers, mine_count);
+            }
+        }
+        else
+        {
+            // the minimum image per pair: 64 centers at a time, lane k holding center first + k
+            // and its count
+            for (index_t first = begin; first < end; first += kWave)
+            {
+                const int centers = end - first < kWave ? end - first : kWave;
+                const V4 mine = pos[lane < centers ? first + lane : first];
+                index_t mine_count = 0;
+                const index_t own = own_first(first);
+                char* rows = reinterpret_cast<char*>(a.index + static_cast<std::int64_t>(first) * a.stride);
+                index_t next_slot, next_local;
+                bool next_valid;
+                candidate(lane, next_slot, next_local, next_valid);
+                V4 next_raw = pos[next_valid ? next_slot : first];
+                for (index_t chunk = 0; chunk < total; chunk += kWave)
+                {
+                    const index_t local = next_local;
+                    const std::uint64_t valid = ballot(next_valid);
+                    const V4 q = next_raw;
+                    if (chunk + kWave < total)
+                    {
+                        candidate(chunk + kWave + lane, next_slot, next_local, next_valid);
+                        next_raw = pos[next_valid ? next_slot : first];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    const index_t rank = chunk + lane - own;
+                    for (int k = 0; k < centers; ++k)
+                    {
+                        const T dx = minimum_image(read_lane(mine.x, k), q.x, lx);
+                        const T dy = minimum_image(read_lane(mine.y, k), q.y, ly);
+                        const T dz = minimum_image(read_lane(mine.z, k), q.z, lz);
+                        const T xx = dx * dx;
+                        const T yy = dy * dy;
+                        const T zz = dz * dz;
+                        const std::uint64_t mask = ballot((xx + yy) + zz < radius_squared) &
+                                                   ballot(rank != k) & valid;
+                        const index_t count = read_lane(mine_count, k);
+                        if (writing) write_hits(mask, count, rows, k, local);
+                        mine_count += lane == k ? static_cast<index_t>(__popcll(mask)) : 0;
+                    }
+                }
+                close_rows(first, centers, mine_count);
+            }
         }
     }
 }
@@ -1653,7 +1770,8 @@ __device__ __forceinline__ void md_unpack_fill_body(const StepArgs<T>& a, const 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) md_unpack_fill_kernel(StepArgs<T> a,
+__global__ void __launch_bounds__(kBlock)
+md_unpack_fill_kernel(StepArgs<T> a,
                                                                 index_t unpack_blocks, T radius,
                                                                 T cull_squared, int forced)
 {
