@@ -706,6 +706,9 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
     __shared__ T block_max[kBlock / kWave];
     const std::int64_t atom = static_cast<std::int64_t>(launch.block) * blockDim.x + threadIdx.x;
     const bool check = a.status[0] == 0;
+    // the box and the reuse rule's scalars, read before any write
+    const T cell[3] = {a.cell[0], a.cell[1], a.cell[2]};
+    const T reuse[4] = {a.reuse[0], a.reuse[1], a.reuse[2], a.reuse[3]};
     bool finite = true;
     T moved = 0, moved_inner = 0;
     if (atom < a.num_atoms)
@@ -713,8 +716,22 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
         T* v = a.velocities + 3 * atom;
         const T* f = a.forces + 3 * atom;
         T* x = a.positions + 3 * atom;
+        // every read of the atom first, then the writes: the compiler cannot tell the arrays
+        // apart, and a read after a write would wait for it (one memory round trip per atom
+        // instead of a chain of them)
         const T inverse_mass = a.inverse_masses[atom];
         T vx = v[0], vy = v[1], vz = v[2];
+        const T f_atom[3] = {f[0], f[1], f[2]};
+        const T x_atom[3] = {x[0], x[1], x[2]};
+        T reference[3] = {0, 0, 0}, at_refresh[3] = {0, 0, 0};
+        if (check)
+        {
+            for (int axis = 0; axis < 3; ++axis)
+                reference[axis] = a.reference_positions[3 * atom + axis];
+            if (a.inner_index)
+                for (int axis = 0; axis < 3; ++axis)
+                    at_refresh[axis] = a.prune_positions[3 * atom + axis];
+        }
         if (a.nvt)
         {
             // the previous step's closing factor, then this step's opening factor
@@ -737,18 +754,18 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
             vz = vz * drag;
             scale = static_cast<T>(a.barostat[kGrowth]);
         }
-        vx = vx + half_timestep * f[0] * inverse_mass;
-        vy = vy + half_timestep * f[1] * inverse_mass;
-        vz = vz + half_timestep * f[2] * inverse_mass;
+        vx = vx + half_timestep * f_atom[0] * inverse_mass;
+        vy = vy + half_timestep * f_atom[1] * inverse_mass;
+        vz = vz + half_timestep * f_atom[2] * inverse_mass;
         T out[3];
         const T vel[3] = {vx, vy, vz};
         std::int64_t crossings[3];
         bool crossed = false;
         for (int axis = 0; axis < 3; ++axis)
         {
-            const T length = a.cell[axis];
-            const T drifted = a.barostat ? (x[axis] * scale + timestep * vel[axis]) * scale
-                                         : x[axis] + timestep * vel[axis];
+            const T length = cell[axis];
+            const T drifted = a.barostat ? (x_atom[axis] * scale + timestep * vel[axis]) * scale
+                                         : x_atom[axis] + timestep * vel[axis];
             T k = floor(drifted / length);
             T w = drifted - k * length;
             if (w < T(0))
@@ -788,14 +805,12 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
             finite = isfinite(out[0]) && isfinite(out[1]) && isfinite(out[2]);
             if (finite)
             {
-                moved = moved_squared(out[0], out[1], out[2], a.reference_positions + 3 * atom,
-                                      a.reuse, a.cell);
+                moved = moved_squared(out[0], out[1], out[2], reference, reuse, cell);
                 if (a.inner_index)
                 {
-                    const T* x_prune = a.prune_positions + 3 * atom;
-                    const T dx = minimum_image(out[0], x_prune[0], a.cell[0]);
-                    const T dy = minimum_image(out[1], x_prune[1], a.cell[1]);
-                    const T dz = minimum_image(out[2], x_prune[2], a.cell[2]);
+                    const T dx = minimum_image(out[0], at_refresh[0], cell[0]);
+                    const T dy = minimum_image(out[1], at_refresh[1], cell[1]);
+                    const T dz = minimum_image(out[2], at_refresh[2], cell[2]);
                     moved_inner = ((dx * dx) + dy * dy) + dz * dz;
                 }
             }
@@ -809,7 +824,7 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
     if (lane_id() == 0)
     {
         block_max[wave] = wave_moved;
-        if (check && wave_moved >= a.reuse[0] && a.rebuild[0] == 0) a.rebuild[0] = 1;
+        if (check && wave_moved >= reuse[0] && a.rebuild[0] == 0) a.rebuild[0] = 1;
         if (check && a.inner_index)
         {
             PruneState* prune = prune_state(a.header);
