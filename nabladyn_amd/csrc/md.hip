@@ -1377,6 +1377,10 @@ struct BrickFrame
     T center[3], length[3];
     bool moved[3];
     bool exact;
+    // placing is the identity for every atom the brick reads: its halo's sub-bins lie at least
+    // one sub-bin inside every face (an atom drifts less than a sub-bin between builds, so none
+    // of them crosses a face), far from the half-box distance, and the frame is not moved
+    bool plain;
 
     __device__ __forceinline__ T image(T p, int axis) const
     {
@@ -1389,7 +1393,7 @@ struct BrickFrame
     template <typename V4>
     __device__ __forceinline__ V4 place(V4 p) const
     {
-        if (!exact) return p;
+        if (!exact || plain) return p;
         p.x = image(p.x, 0);
         p.y = image(p.y, 1);
         p.z = image(p.z, 2);
@@ -1421,6 +1425,13 @@ __device__ __forceinline__ BrickFrame<T> brick_frame(const StepArgs<T>& a, const
         const bool upper = !f.moved[axis] || f.center[axis] - extent > T(0.5) * length;
         f.exact = f.exact && inside && upper;
     }
+    // the halo's sub-bins [lo - 2, hi + 2) with a sub-bin more inside every face, and the whole
+    // span plus that margin within half the box
+    const index_t dims[3] = {a.nx, a.ny, a.nz};
+    f.plain = f.exact;
+    for (int axis = 0; axis < 3; ++axis)
+        f.plain = f.plain && !f.moved[axis] && box.lo[axis] >= 3 &&
+                  box.hi[axis] + 3 <= dims[axis] && box.hi[axis] - box.lo[axis] + 6 < dims[axis];
     return f;
 }
 
