@@ -2061,28 +2061,17 @@ __device__ __forceinline__ StagedBrick<T> stage_brick(const StepArgs<T>& a, inde
 // Lanes per row of the prune kernel: 16, each reading 4 entries per pass.
 constexpr int kPruneGroup = 16;
 
-// Refreshes the pruned list from the full rows, one block per brick (the halo in LDS): 16 lanes
-// per row test the row's pairs against the pruned radius with the exact minimum-image expression
-// and write those kept from the row's start in pass order (ballot ranks, so each store is one
-// contiguous run per row), padded with the sentinel 0 up to a multiple of 32; the count is the
-// number kept. Since every pair kept is in the full row, a pruned row never outgrows the stride.
-// The frame of the refresh is kept for the check. Gated on the refresh flag unless `forced`.
-template <typename T>
-__device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch launch,
-                                              int forced)
+// Refreshes the pruned rows of a brick staged with images exact out to the pruned radius (the
+// halo in LDS), by its block: 16 lanes per row test the row's pairs against the pruned radius
+// with the exact minimum-image expression and write those kept from the row's start in pass
+// order (ballot ranks, so each store is one contiguous run per row), padded with the sentinel 0
+// up to a multiple of 32; the count is the number kept. Since every pair kept is in the full
+// row, a pruned row never outgrows the stride. The frame of the refresh is kept for the check.
+template <typename T, typename V4>
+__device__ __forceinline__ void prune_brick_rows(const StepArgs<T>& a,
+                                                 const StagedBrick<T>& staged, const V4* halo)
 {
 #pragma clang fp contract(off)
-    using V4 = typename Vec4<T>::type;
-    if (!forced && !refreshing(a)) return;
-    extern __shared__ __align__(16) unsigned char dynamic_shared[];
-    V4* halo = reinterpret_cast<V4*>(dynamic_shared);
-    BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
-    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks), launch);
-    if (brick >= a.bricks) return;
-    if (launch.block == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
-    // images exact out to the pruned radius
-    const StagedBrick<T> staged =
-        stage_brick<kBrickThreads<T>>(a, brick, halo, sh, a.inner_radius);
     const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const T inner_squared = a.inner_radius * a.inner_radius;
@@ -2179,6 +2168,26 @@ __device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch
     }
 }
 
+// The refresh of every brick's pruned rows, one block per brick; gated on the refresh flag
+// unless `forced`.
+template <typename T>
+__device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch launch,
+                                              int forced)
+{
+    using V4 = typename Vec4<T>::type;
+    if (!forced && !refreshing(a)) return;
+    extern __shared__ __align__(16) unsigned char dynamic_shared[];
+    V4* halo = reinterpret_cast<V4*>(dynamic_shared);
+    BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
+    const index_t brick = xcd_brick(static_cast<index_t>(a.bricks), launch);
+    if (brick >= a.bricks) return;
+    if (launch.block == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
+    // images exact out to the pruned radius
+    const StagedBrick<T> staged =
+        stage_brick<kBrickThreads<T>>(a, brick, halo, sh, a.inner_radius);
+    prune_brick_rows(a, staged, halo);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBrickThreads<T>) md_prune_kernel(StepArgs<T> a, int forced)
 {
@@ -2206,7 +2215,8 @@ constexpr int kForceWaves = kVirial ? 1 : Pot::template waves<T>;
 template <typename T, class Pot, int G, bool kVirial, bool kKick>
 __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch launch,
                                            const Pot& pot, T cutoff, T cutoff_squared,
-                                           T half_timestep, double* __restrict__ partials)
+                                           T half_timestep, double* __restrict__ partials,
+                                           int refresh = 0)
 {
     using V4 = typename Vec4<T>::type;
     constexpr bool kEnergy = !kKick || kVirial;
@@ -2228,8 +2238,22 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
     // the potential's own LDS after the brick's (the halo staging's barrier covers it)
     unsigned char* potential_shared = dynamic_shared + brick_block_bytes<T>(a.halo_capacity);
     stage_potential(pot, potential_shared);
-    // images exact out to the cutoff
-    const StagedBrick<T> staged = stage_brick<kBrickThreads<T>>(a, brick, halo, sh, cutoff);
+    // a step that refreshes the pruned list refreshes this brick's rows first (the rows the walk
+    // reads are the brick's own), on the same staged halo. A step that may refresh stages images
+    // exact out to the pruned radius, so the staging need not wait for the flag (a pair within
+    // the cutoff gets the same displacement in either frame); the others out to the cutoff.
+    const bool may_refresh = refresh && pruned;
+    const bool refreshes = may_refresh && refreshing(a);
+    const StagedBrick<T> staged = stage_brick<kBrickThreads<T>>(
+        a, brick, halo, sh, may_refresh ? static_cast<T>(a.inner_radius) : cutoff);
+    if (refreshes && launch.block == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
+    if (refreshes)
+    {
+        prune_brick_rows(a, staged, halo);
+        // the rows written, seen by every thread of the block (the barrier orders the block's
+        // memory; no other block reads these rows)
+        __syncthreads();
+    }
     const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const int sub = lane_id() % G;
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
@@ -2312,10 +2336,10 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
 template <typename T, class Pot, int G, bool kVirial, bool kKick>
 __global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial>))
     force_kernel(StepArgs<T> a, Pot pot, T cutoff, T cutoff_squared, T half_timestep,
-                 double* __restrict__ partials)
+                 double* __restrict__ partials, int refresh)
 {
     force_body<T, Pot, G, kVirial, kKick>(a, own_launch(), pot, cutoff, cutoff_squared,
-                                          half_timestep, partials);
+                                          half_timestep, partials, refresh);
 }
 
 // The rows of the list as slots, for readout: out[slot * stride + k] = the slot's k-th listed
@@ -2638,51 +2662,53 @@ void allow_shared(Kernel kernel, std::size_t bytes)
 // The force launch of one schedule.
 template <typename T, class Pot, int G, bool kVirial, bool kKick>
 void launch_force_kernel(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
-                         T half_timestep, double* partials, unsigned grid, hipStream_t stream)
+                         T half_timestep, double* partials, unsigned grid, hipStream_t stream,
+                         int refresh)
 {
     const std::size_t lds = brick_block_bytes<T>(a.halo_capacity) + potential_lds(pot);
     allow_shared(force_kernel<T, Pot, G, kVirial, kKick>, lds);
     force_kernel<T, Pot, G, kVirial, kKick><<<grid, kBrickThreads<T>, lds, stream>>>(
-        a, pot, cutoff, cutoff_squared, half_timestep, partials);
+        a, pot, cutoff, cutoff_squared, half_timestep, partials, refresh);
 }
 
 template <typename T, class Pot, int G>
 void launch_force_g(const StepArgs<T>& a, const Pot& pot, T cutoff, T cutoff_squared,
                     T half_timestep, bool virial, bool kick, double* partials, unsigned grid,
-                    hipStream_t stream)
+                    hipStream_t stream, int refresh)
 {
     if (virial && kick)
         launch_force_kernel<T, Pot, G, true, true>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                                   partials, grid, stream);
+                                                   partials, grid, stream, refresh);
     else if (kick)
         launch_force_kernel<T, Pot, G, false, true>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                                    partials, grid, stream);
+                                                    partials, grid, stream, refresh);
     else if (virial)
         launch_force_kernel<T, Pot, G, true, false>(a, pot, cutoff, cutoff_squared, half_timestep,
-                                                    partials, grid, stream);
+                                                    partials, grid, stream, refresh);
     else
         launch_force_kernel<T, Pot, G, false, false>(a, pot, cutoff, cutoff_squared,
-                                                     half_timestep, partials, grid, stream);
+                                                     half_timestep, partials, grid, stream,
+                                                     refresh);
 }
 
 template <typename T, class Pot>
 void launch_force(const StepArgs<T>& a, const Pot& pot, int group, T cutoff, T cutoff_squared,
                   T half_timestep, bool virial, bool kick, double* partials, unsigned grid,
-                  hipStream_t stream)
+                  hipStream_t stream, int refresh)
 {
     switch (group)
     {
         case 4:
             launch_force_g<T, Pot, 4>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
-                                      partials, grid, stream);
+                                      partials, grid, stream, refresh);
             break;
         case 8:
             launch_force_g<T, Pot, 8>(a, pot, cutoff, cutoff_squared, half_timestep, virial, kick,
-                                      partials, grid, stream);
+                                      partials, grid, stream, refresh);
             break;
         default:
             launch_force_g<T, Pot, 16>(a, pot, cutoff, cutoff_squared, half_timestep, virial,
-                                       kick, partials, grid, stream);
+                                       kick, partials, grid, stream, refresh);
     }
 }
 
@@ -2782,7 +2808,7 @@ bool stages_table(const ndn_potential* pot, std::int64_t halo_capacity)
 template <typename T>
 int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coefficients, int group,
                   bool virial, bool kick, double* partials, std::int64_t partial_slots,
-                  hipStream_t stream, int* grid_out)
+                  hipStream_t stream, int* grid_out, int refresh = 0)
 {
     NDN_REQUIRE(valid_group(group), "group must be 4, 8 or 16");
     const unsigned grid = force_grid(a.bricks);
@@ -2803,24 +2829,25 @@ int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coeffici
         const PairSum<T> sum{sum_part<T>(*pot, coefficients),
                              sum_part<T>(*pot->next, second_coefficients<T>(pot, coefficients))};
         launch_force<T>(a, sum, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                        partials, grid, stream);
+                        partials, grid, stream, refresh);
     }
     else if (pot->kind == NDN_LJ && pot->num_types == 1)
     {
         const OneTypeLJ<T> lj{static_cast<const LJRow<T>*>(coefficients)};
         launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                        partials, grid, stream);
+                        partials, grid, stream, refresh);
     }
     else if (pot->kind == NDN_LJ)
     {
         const TypedLJ<T> lj{static_cast<const LJRow<T>*>(coefficients), pot->num_types};
         launch_force<T>(a, lj, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                        partials, grid, stream);
+                        partials, grid, stream, refresh);
     }
     else if (stages_table<T>(pot, a.halo_capacity))
     {
         launch_force<T>(a, bind_one_type_table<T>(pot, coefficients), group, cutoff,
-                        cutoff_squared, half_timestep, virial, kick, partials, grid, stream);
+                        cutoff_squared, half_timestep, virial, kick, partials, grid, stream,
+                        refresh);
     }
     else
     {
@@ -2830,7 +2857,7 @@ int launch_forces(const StepArgs<T>& a, const ndn_potential* pot, void* coeffici
             padded + static_cast<std::int64_t>(rows) * (pot->knots + 2), nullptr,
             pot->num_types, static_cast<int>(pot->knots)}};
         launch_force<T>(a, table, group, cutoff, cutoff_squared, half_timestep, virial, kick,
-                        partials, grid, stream);
+                        partials, grid, stream, refresh);
     }
     NDN_LAUNCHED("md_force");
     *grid_out = static_cast<int>(grid);
@@ -2866,11 +2893,11 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     NDN_LAUNCHED("md_first_half");
     const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream);
     if (rebuilt != NDN_OK) return rebuilt;
-    const int pruned = launch_prune<T>(a, 0, stream);
-    if (pruned != NDN_OK) return pruned;
+    // with a pruned list, each force block refreshes its brick's pruned rows when the step
+    // refreshes them (a build, or the check's flag), then walks them
     int blocks = 0;
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, settings->virial != 0,
-                                        true, s.partials, s.partial_slots, stream, &blocks);
+                                        true, s.partials, s.partial_slots, stream, &blocks, 1);
     if (forces != NDN_OK) return forces;
     finish_step_kernel<T><<<1, kFinishThreads, 0, stream>>>(a, s.partials, blocks, 1);
     NDN_LAUNCHED("md_finish");
