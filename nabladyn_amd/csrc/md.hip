@@ -1221,34 +1221,61 @@ __device__ __forceinline__ void md_halo_body(const StepArgs<T>& a, const Launch 
     }
 }
 
-// The build's exclusive scan of the sub-bin counts into bin_offsets (bins + 1 values) by one
-// block. A pass takes 16 rows of 1024 consecutive values, thread t holding value t of each row
-// (every load and store coalesced): each row is scanned within its waves, one wave scans the
-// pass's 16 x 16 wave totals in order, and every value adds its row's and wave's prefix; the
-// running total carries from pass to pass. One launch where a tiled scan takes three: the steps
-// without a build, nearly all, pay one empty launch.
+// The build's exclusive scan of the sub-bin counts into bin_offsets (bins + 1 values). A pass
+// takes 16 rows of 1024 consecutive values, thread t holding value t of each row (every load and
+// store coalesced): each row is scanned within its waves, one wave scans the pass's 16 x 16 wave
+// totals in order, and every value adds its row's and wave's prefix; the running total carries
+// from pass to pass. Each block of the launch takes its own run of passes and first sums every
+// count before its run itself (integer sums: the same bits in any order), so the blocks need
+// nothing from each other: one launch where a tiled scan takes three, and the steps without a
+// build, nearly all, pay one empty launch.
 constexpr int kListScanThreads = 1024;
 constexpr int kListScanRows = 16;
 constexpr int kListScanWaves = kListScanThreads / kWave;
+constexpr std::int64_t kListScanPass = static_cast<std::int64_t>(kListScanThreads) * kListScanRows;
+
+// Blocks of the scan of `bins` sub-bins: one per pass.
+inline std::int64_t scan_blocks(std::int64_t bins)
+{
+    return (bins + 1 + kListScanPass - 1) / kListScanPass;
+}
 
 template <typename T>
-__device__ __forceinline__ void md_scan_body(const StepArgs<T>& a)
+__device__ __forceinline__ void md_scan_body(const StepArgs<T>& a, const Launch launch)
 {
     // [row][wave] totals, scanned in place; then the pass total
     __shared__ index_t totals[kListScanRows * kListScanWaves + 1];
     const int lane = lane_id();
     const int wave = threadIdx.x / kWave;
     const std::int64_t count = a.bins + 1;
-    constexpr std::int64_t kPass = static_cast<std::int64_t>(kListScanThreads) * kListScanRows;
+    constexpr std::int64_t kPass = kListScanPass;
+    const std::int64_t passes = (count + kPass - 1) / kPass;
+    const std::int64_t per_block = (passes + launch.blocks - 1) / launch.blocks;
+    const std::int64_t begin = launch.block * per_block * kPass;
+    if (begin >= count) return;
+    const std::int64_t end = begin + per_block * kPass < count ? begin + per_block * kPass : count;
+    // the counts before this block's run: 4 at a time (the run starts at a multiple of 16384)
+    index_t before = 0;
+    const int4* counts4 = reinterpret_cast<const int4*>(a.bin_counts);
+    for (std::int64_t q = threadIdx.x; q < begin / 4; q += kListScanThreads)
+    {
+        const int4 c = counts4[q];
+        before += (c.x + c.y) + (c.z + c.w);
+    }
+    before = wave_inclusive_sum(before);
+    if (lane == kWave - 1) totals[wave] = before;
+    __syncthreads();
     index_t carry = 0;
-    for (std::int64_t base = 0; base < count; base += kPass)
+    for (int w = 0; w < kListScanWaves; ++w) carry += totals[w];
+    __syncthreads();
+    for (std::int64_t base = begin; base < end; base += kPass)
     {
         index_t v[kListScanRows], inclusive[kListScanRows];
 #pragma unroll
         for (int j = 0; j < kListScanRows; ++j)
         {
             const std::int64_t at = base + static_cast<std::int64_t>(j) * kListScanThreads + threadIdx.x;
-            v[j] = at < count ? a.bin_counts[at] : 0;
+            v[j] = at < end ? a.bin_counts[at] : 0;
         }
 #pragma unroll
         for (int j = 0; j < kListScanRows; ++j) inclusive[j] = wave_inclusive_sum(v[j]);
@@ -1282,7 +1309,7 @@ __device__ __forceinline__ void md_scan_body(const StepArgs<T>& a)
         for (int j = 0; j < kListScanRows; ++j)
         {
             const std::int64_t at = base + static_cast<std::int64_t>(j) * kListScanThreads + threadIdx.x;
-            if (at < count)
+            if (at < end)
                 a.bin_offsets[at] = carry + totals[j * kListScanWaves + wave] + inclusive[j] - v[j];
         }
         carry += totals[kListScanRows * kListScanWaves];
@@ -1294,7 +1321,7 @@ template <typename T>
 __global__ void __launch_bounds__(kListScanThreads) md_scan_kernel(StepArgs<T> a, int forced)
 {
     if (!forced && !rebuilding(a.rebuild, a.status)) return;
-    md_scan_body(a);
+    md_scan_body(a, own_launch());
 }
 
 // The scatter and the halo listing, both after the scan, in one launch: the first
@@ -1344,7 +1371,7 @@ __global__ void __launch_bounds__(kListScanThreads) md_build_small_kernel(StepAr
     md_assign_body(a, one, 1);
     __syncthreads();
     if (!gate()) return;
-    md_scan_body(a);
+    md_scan_body(a, Launch{0, 1});
     __syncthreads();
     if (!gate()) return;
     md_scatter_body(a, one, 1);
@@ -2602,7 +2629,8 @@ int launch_md_rebuild(const StepArgs<T>& a, double radius, int forced, hipStream
     }
     md_assign_kernel<T><<<gated_grid(n), kBlock, 0, stream>>>(a, forced);
     NDN_LAUNCHED("md_assign");
-    md_scan_kernel<T><<<1, kListScanThreads, 0, stream>>>(a, forced);
+    md_scan_kernel<T><<<static_cast<unsigned>(scan_blocks(a.bins)), kListScanThreads, 0, stream>>>(
+        a, forced);
     NDN_LAUNCHED("md_scan");
     const unsigned scatter = gated_grid(n);
     const unsigned halo = gated_grid(a.bricks, kBlock / kWave);
@@ -3152,7 +3180,7 @@ __global__ void __launch_bounds__(kListScanThreads) batch_scan_kernel(BatchTable
     const BatchBlock b = batch_block(t.map, t.blocks, kPartScan);
     const StepArgs<T> a = t.members[b.member].a;
     if (!rebuilding(a.rebuild, a.status)) return;
-    md_scan_body(a);
+    md_scan_body(a, b.launch);
 }
 
 template <typename T>
@@ -3252,7 +3280,7 @@ void member_blocks(const StepArgs<T>& a, index_t out[kBatchParts])
     out[kPartPrologue] = 1;
     out[kPartFirstHalf] = static_cast<index_t>(blocks_for(n));
     out[kPartAssign] = static_cast<index_t>(gated_grid(n));
-    out[kPartScan] = 1;
+    out[kPartScan] = static_cast<index_t>(scan_blocks(a.bins));
     out[kPartScatter] = static_cast<index_t>(gated_grid(n));
     out[kPartHalo] = static_cast<index_t>(gated_grid(a.bricks, kBlock / kWave));
     out[kPartOrder] = static_cast<index_t>(gated_grid(n));
