@@ -252,8 +252,10 @@ int ndn_pair_efw_noalloc(int dtype, const void* positions, const void* cell, int
 
 /* The resident frame of a simulation (md/_state.py:8-21), kept in cell order ("slots"): every
  * per-atom array below is permuted at each rebuild, and ndn_md_list.ids maps a slot to its atom.
- * thermostat [8] f64: chain positions [3], momenta [3], the closing velocity factor still to be
- * applied (1 when settled), the current opening factor. kinetic [1] f64 is the K the chain
+ * thermostat [14] f64: chain positions [3], momenta [3], the closing velocity factor still to be
+ * applied (1 when settled), the opening factor of the next step, and the chain after that step's
+ * opening half step (positions [3], momenta [3]; a step's finish commits them before its closing
+ * half step, then opens the next step). kinetic [1] f64 is the K the chain
  * reads (md/_ensembles/_nvt.py:47-49). partials [partial_slots * 8] f64 and coefficients
  * [coefficient_bytes] are scratch sized by ndn_md_workspace. virial may be NULL. */
 typedef struct ndn_md_system {
@@ -267,7 +269,7 @@ typedef struct ndn_md_system {
     const void* cell;           /* [3] T */
     void* energy;               /* [1] T */
     void* virial;               /* [3, 3] T or NULL */
-    double* thermostat;         /* [8] */
+    double* thermostat;         /* [14] */
     const double* chain_masses; /* [3]: g kT tau^2, kT tau^2, kT tau^2 */
     double* kinetic;            /* [1] */
     double* partials;
@@ -289,6 +291,10 @@ typedef struct ndn_md_settings {
     double pressure;              /* kJ/mol/nm^3 */
     double box_mass;              /* 3 (N + 1) kT tau_b^2 */
     double alpha;                 /* 1 + 1/N */
+    /* 1: this step follows a step of the same run on the stream with nothing in between (a
+     * captured run's later steps), so it skips the prologue launch: the previous step's finish
+     * opened it. Without a barostat only; 0 for a first step. */
+    int continues;
 } ndn_md_settings;
 
 /* The reuse state of a neighbor list (neighbors/_neighbor_list.py:42-56). */

@@ -47,11 +47,15 @@
 namespace ndn {
 
 // Thermostat slots (fp64): chain positions [0..2], momenta [3..5], the pending closing factor
-// [6], the opening factor of the current step [7].
+// [6], the opening factor of the next (or current) step [7], and the chain after that step's
+// opening half step, positions [8..10] and momenta [11..13], which the step's finish commits
+// before its closing half step (without a barostat).
 constexpr int kChainQ = 0;
 constexpr int kChainP = 3;
 constexpr int kPending = 6;
 constexpr int kOpening = 7;
+constexpr int kOpenQ = 8;
+constexpr int kOpenP = 11;
 constexpr int kStencil = 5;          // sub-bins per axis a stencil spans
 
 // Barostat slots (fp64, NPT only): the box momentum p_eps [0], its chain's positions [1..3] and
@@ -602,6 +606,56 @@ __host__ __device__ inline ndn_potential no_second_model()
     return none;
 }
 
+// The list check's preparation for a step, by one thread: the reuse rule's (prepare_update) and
+// the pruned list's threshold, the outer rule's allowance at the pruned list's radius (the cell
+// is fixed between refreshes; none left means a refresh every step).
+template <typename T>
+__device__ __forceinline__ void prepare_checks(const StepArgs<T>& a, bool reset_moved)
+{
+    prepare_update(a.cell, a.reference_cell, a.cutoff, a.radius, a.limits, a.dims_scratch,
+                   a.widths_scratch, a.reuse, a.rebuild, a.status, a.total_scratch, a.header,
+                   reset_moved);
+    if (a.inner_index)
+    {
+#pragma clang fp contract(off)
+        const T epsilon = Epsilon<T>::value;
+        T longest = a.cell[0];
+        for (int axis = 1; axis < 3; ++axis) longest = a.cell[axis] > longest ? a.cell[axis] : longest;
+        const T allowance = a.inner_radius - a.cutoff - 32 * epsilon * longest;
+        PruneState* prune = prune_state(a.header);
+        prune->threshold = allowance > 0 ? static_cast<double>(allowance * allowance / 4) : -1.0;
+        prune->refresh = 0;
+    }
+}
+
+// The opening of a step without a barostat, by one thread: the checks' preparation and the
+// chain's opening half step on a copy of the chain: its velocity factor, and the opened chain in
+// the thermostat's opened slots for the step's finish to commit. Nothing in the step reads the
+// chain or K before the finish, so the previous step's finish can open the next step (and a
+// step that continues a run needs no prologue launch).
+template <typename T>
+__device__ __forceinline__ void open_step(const StepArgs<T>& a, bool reset_moved)
+{
+    if (a.nvt)
+    {
+        double* t = a.thermostat;
+        double q[3], p[3];
+        double kinetic = a.kinetic[0];
+        for (int k = 0; k < 3; ++k)
+        {
+            q[k] = t[kChainQ + k];
+            p[k] = t[kChainP + k];
+        }
+        t[kOpening] = chain_half_step(&kinetic, q, p, a.chain_masses, a.degrees, a.kT, a.timestep);
+        for (int k = 0; k < 3; ++k)
+        {
+            t[kOpenQ + k] = q[k];
+            t[kOpenP + k] = p[k];
+        }
+    }
+    prepare_checks(a, reset_moved);
+}
+
 // `second` is the second model of a sum (kind -1 without one), its table at second_table.
 template <typename T>
 __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_potential& pot,
@@ -610,30 +664,21 @@ __device__ __forceinline__ void prologue_body(const StepArgs<T>& a, const ndn_po
 {
     if (threadIdx.x == 0)
     {
-        // NPT: the barostat's chain, the particles' chain, then the box (_npt.py:154-174)
-        if (a.barostat) box_chain_half_step(a);
-        if (a.nvt)
+        if (a.barostat)
         {
-            double* t = a.thermostat;
-            t[kOpening] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
-                                          a.degrees, a.kT, a.timestep);
+            // NPT: the barostat's chain, the particles' chain, then the box (_npt.py:154-174)
+            box_chain_half_step(a);
+            if (a.nvt)
+            {
+                double* t = a.thermostat;
+                t[kOpening] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP,
+                                              a.chain_masses, a.degrees, a.kT, a.timestep);
+            }
+            box_first_half(a);
+            prepare_checks(a, true);
         }
-        if (a.barostat) box_first_half(a);
-        prepare_update(a.cell, a.reference_cell, a.cutoff, a.radius, a.limits, a.dims_scratch,
-                       a.widths_scratch, a.reuse, a.rebuild, a.status, a.total_scratch, a.header);
-        if (a.inner_index)
-        {
-            // the outer rule's allowance with the pruned list's radius (the cell is fixed
-            // between refreshes); none left means a refresh every step
-#pragma clang fp contract(off)
-            const T epsilon = Epsilon<T>::value;
-            T longest = a.cell[0];
-            for (int axis = 1; axis < 3; ++axis) longest = a.cell[axis] > longest ? a.cell[axis] : longest;
-            const T allowance = a.inner_radius - a.cutoff - 32 * epsilon * longest;
-            PruneState* prune = prune_state(a.header);
-            prune->threshold = allowance > 0 ? static_cast<double>(allowance * allowance / 4) : -1.0;
-            prune->refresh = 0;
-        }
+        else
+            open_step(a, true);
     }
     write_coefficients<T>(pot, coefficients);
     if (second.kind >= 0) write_coefficients<T>(second, second_table);
@@ -2419,6 +2464,20 @@ __device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
                                                  int kinetic_from_partials)
 {
     __shared__ double column[kFinishThreads];
+    // without a barostat, thread 0 reads the chain and its masses while the partials are summed
+    const bool chain_in_registers =
+        threadIdx.x == 0 && kinetic_from_partials && a.nvt && !a.barostat;
+    double chain[kOpenP + 3], masses[3];
+    if (chain_in_registers)
+    {
+        for (int k = 0; k < kOpenP + 3; ++k) chain[k] = a.thermostat[k];
+        for (int k = 0; k < 3; ++k) masses[k] = a.chain_masses[k];
+    }
+    // without a barostat, the next step's checks, by another wave meanwhile (nothing here reads
+    // what they write); with the chain's opening below, the next step's prologue then only
+    // writes the potential's coefficients, which a step continuing a run already has. The
+    // record of the largest displacement is kept for reading until the next prologue.
+    if (threadIdx.x == kWave && kinetic_from_partials && !a.barostat) prepare_checks(a, false);
     // thread t sums partial (t % 8) of blocks t / 8, t / 8 + 128, ...: coalesced 8-byte reads,
     // 16 in flight per thread (two rounds of reads at 4,000 blocks), each sum in a fixed order
     constexpr int kInFlight = 16;
@@ -2473,7 +2532,36 @@ __device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
 #pragma clang fp contract(off)
         a.barostat[kBoxP] = a.barostat[kBoxP] + (0.5 * a.timestep) * box_force(a);
     }
-    if (a.nvt && kinetic_from_partials)
+    if (chain_in_registers)
+    {
+        // the step's opening half step (it waits in the opened slots), the closing half step,
+        // and the next step's opening on a copy, in registers; then every slot written once
+        double kinetic = sums[7];
+        double q[3], p[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            q[k] = chain[kOpenQ + k];
+            p[k] = chain[kOpenP + k];
+        }
+        const double closing =
+            chain_half_step(&kinetic, q, p, masses, a.degrees, a.kT, a.timestep);
+        double next_kinetic = kinetic;
+        double next_q[3] = {q[0], q[1], q[2]}, next_p[3] = {p[0], p[1], p[2]};
+        const double opening =
+            chain_half_step(&next_kinetic, next_q, next_p, masses, a.degrees, a.kT, a.timestep);
+        double* t = a.thermostat;
+        for (int k = 0; k < 3; ++k)
+        {
+            t[kChainQ + k] = q[k];
+            t[kChainP + k] = p[k];
+            t[kOpenQ + k] = next_q[k];
+            t[kOpenP + k] = next_p[k];
+        }
+        t[kPending] = closing;
+        t[kOpening] = opening;
+        a.kinetic[0] = kinetic;
+    }
+    else if (a.nvt && kinetic_from_partials)
     {
         double* t = a.thermostat;
         t[kPending] = chain_half_step(a.kinetic, t + kChainQ, t + kChainP, a.chain_masses,
@@ -2914,9 +3002,13 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     const T timestep = static_cast<T>(settings->timestep);
     const T half_timestep = static_cast<T>(0.5 * settings->timestep);
     const ndn_potential second = pot->next != nullptr ? *pot->next : no_second_model();
-    prologue_kernel<T><<<1, kBlock, 0, stream>>>(a, *pot, s.coefficients, second,
-                                                 second_coefficients<T>(pot, s.coefficients));
-    NDN_LAUNCHED("md_prologue");
+    // a step continuing a run without a barostat was opened by the previous step's finish
+    if (!settings->continues || s.barostat != nullptr)
+    {
+        prologue_kernel<T><<<1, kBlock, 0, stream>>>(a, *pot, s.coefficients, second,
+                                                     second_coefficients<T>(pot, s.coefficients));
+        NDN_LAUNCHED("md_prologue");
+    }
     first_half_kernel<T><<<blocks_for(s.num_atoms), kBlock, 0, stream>>>(a, timestep, half_timestep);
     NDN_LAUNCHED("md_first_half");
     const int rebuilt = launch_md_rebuild<T>(a, list->radius, 0, stream);

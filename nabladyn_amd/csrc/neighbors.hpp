@@ -30,19 +30,21 @@ __device__ __forceinline__ T minimum_image(T center, T other, T length)
 // The update's preparation (update/update.cuh:27-75): resets the rebuild flag, checks the
 // cell, chooses the grid (at most `limits` bins per axis, each wider than the radius) and sets
 // the squared reuse threshold and per-axis scales; records the previous pair total for the
-// padding. One thread.
+// padding. One thread. `reset_moved` clears the check's record of the largest displacement.
 template <typename T>
 __device__ __forceinline__ void prepare_update(const T* __restrict__ cell,
                                                const T* __restrict__ reference_cell, T cutoff,
                                                T radius, const index_t* __restrict__ limits,
                                                index_t* dims, T* widths, T* reuse,
                                                index_t* rebuild, index_t* status,
-                                               const index_t* pair_total, ListHeader* header)
+                                               const index_t* pair_total, ListHeader* header,
+                                               bool reset_moved = true)
 {
 #pragma clang fp contract(off)
     const T epsilon = Epsilon<T>::value;
     header->old_total = pair_total[0];
-    for (int k = 0; k < kMovedSlots; ++k) *moved_slot(header, k) = 0ull;
+    if (reset_moved)
+        for (int k = 0; k < kMovedSlots; ++k) *moved_slot(header, k) = 0ull;
     rebuild[0] = 0;
     if (status[0] != 0) return;
     T smallest_scale = cell[0] / reference_cell[0];

@@ -59,6 +59,7 @@ class MDSettings(_native.ctypes.Structure):
         ("degrees_of_freedom", _native.ctypes.c_int64), ("thermostat", _native.ctypes.c_int),
         ("virial", _native.ctypes.c_int), ("pressure", _native.ctypes.c_double),
         ("box_mass", _native.ctypes.c_double), ("alpha", _native.ctypes.c_double),
+        ("continues", _native.ctypes.c_int),
     ]
 
 
@@ -307,7 +308,8 @@ class _MDNeighbors:
         return int(self.buffers["counts"].sum(dtype=torch.int64).item()) // 2
 
     def _max_moved(self) -> float:
-        """The largest squared displacement measured at the last reuse check, in nm^2."""
+        """The largest squared displacement measured at the reuse checks since the last prologue
+        (a step's own, or the first step of a captured run), in nm^2."""
         raw = self.buffers["scan_temp"][64:64 + 8 * 64].view(torch.int64).max().item()
         import struct
 
@@ -494,7 +496,9 @@ class Simulation:
         self._masses = masses
         self.energy = torch.zeros((), dtype=dtype, device=device)
         self.virial = torch.zeros((3, 3), dtype=dtype, device=device)
-        self.thermostat = torch.zeros(8, **wide)
+        # chain q [3], p [3], the pending closing factor, the next opening factor, and the
+        # chain after the next step's opening half step (q [3], p [3])
+        self.thermostat = torch.zeros(14, **wide)
         self.thermostat[6] = 1.0
         self.thermostat[7] = 1.0
         self.kinetic = torch.zeros(1, **wide)
@@ -684,11 +688,15 @@ class Simulation:
                      _native.ctypes.byref(self._system), _native.stream(self._device))
 
     def _capture(self, steps: int = 1) -> torch.cuda.CUDAGraph:
+        """A graph of `steps` steps: the first with its prologue, the others opened by the step
+        before them (each launch reads the settings when it is recorded)."""
         graph = torch.cuda.CUDAGraph()
         torch.cuda.synchronize(self._device)
         with torch.cuda.graph(graph):
-            for _ in range(steps):
+            for k in range(steps):
+                self._settings.continues = 1 if k > 0 else 0
                 self._launch_step()
+        self._settings.continues = 0
         torch.cuda.synchronize(self._device)
         return graph
 
