@@ -922,6 +922,34 @@ __device__ __forceinline__ std::int64_t row_first_bin(index_t y, index_t z, inde
             static_cast<std::int64_t>(y) * depth + layer);
 }
 
+// Runs of equal keys in neighbouring lanes (the active lanes a prefix of the wave, as in a
+// grid-stride loop over consecutive slots): whether this lane starts its run, the run's length
+// (for a starting lane) and the run's first lane (for every lane). The slots of a frame sorted
+// at the last build stay mostly in their sub-bins, so a wave's atoms come in runs of one
+// sub-bin, and one atomic per run stands for the run's atoms.
+struct LaneRun
+{
+    bool head;
+    int length;
+    int first;
+};
+
+__device__ __forceinline__ LaneRun equal_runs(index_t key)
+{
+    const int lane = lane_id();
+    const std::uint64_t active = ballot(true);
+    const index_t before = __shfl_up(key, 1, kWave);
+    LaneRun run;
+    run.head = lane == 0 || before != key;
+    // inactive lanes end runs too
+    const std::uint64_t heads = ballot(run.head) | ~active;
+    const std::uint64_t higher = lane == kWave - 1 ? 0ull : heads >> (lane + 1);
+    run.length = (higher == 0 ? kWave : lane + 1 + __builtin_ctzll(higher)) - lane;
+    const std::uint64_t lower = lane == kWave - 1 ? heads : heads & ((2ull << lane) - 1ull);
+    run.first = 63 - __builtin_clzll(lower);
+    return run;
+}
+
 template <typename T>
 __device__ __forceinline__ void md_assign_body(const StepArgs<T>& a, const Launch launch,
                                                int forced)
@@ -936,7 +964,8 @@ __device__ __forceinline__ void md_assign_body(const StepArgs<T>& a, const Launc
             sub_bin(p[0], w[0], a.nx) + row_first_bin(sub_bin(p[1], w[1], a.ny),
                                                       sub_bin(p[2], w[2], a.nz), a.nx, a.ny, a.nz));
         a.bin_of[slot] = bin;
-        atomicAdd(a.bin_counts + bin, 1);
+        const LaneRun run = equal_runs(bin);
+        if (run.head) atomicAdd(a.bin_counts + bin, run.length);
     }
 }
 
@@ -955,7 +984,14 @@ __device__ __forceinline__ void md_scatter_body(const StepArgs<T>& a, const Laun
          slot < a.num_atoms; slot += static_cast<std::int64_t>(launch.blocks) * blockDim.x)
     {
         const index_t bin = a.bin_of[slot];
-        a.unordered[a.bin_offsets[bin] + atomicSub(a.bin_counts + bin, 1) - 1] =
+        // one atomic per run of a sub-bin's atoms; the run's lanes take the places below the
+        // count it returns, one each (the order kernel ranks a sub-bin's atoms by id after)
+        const LaneRun run = equal_runs(bin);
+        index_t count = 0;
+        if (run.head) count = atomicSub(a.bin_counts + bin, run.length);
+        count = __shfl(count, run.first, kWave);
+        const int lane = lane_id();
+        a.unordered[a.bin_offsets[bin] + count - 1 - (lane - run.first)] =
             static_cast<index_t>(slot);
     }
 }
@@ -985,13 +1021,9 @@ __device__ __forceinline__ void md_order_body(const StepArgs<T>& a, const Launch
     {
         const std::int64_t old = a.unordered[i];
         const index_t bin = a.bin_of[old];
-        const index_t begin = a.bin_offsets[bin];
-        const index_t end = a.bin_offsets[bin + 1];
         const index_t id = a.ids[old];
-        index_t rank = 0;
-        for (index_t k = begin; k < end; ++k) rank += a.ids[a.unordered[k]] < id;
-        const std::int64_t slot = begin + rank;
-        a.order[slot] = static_cast<index_t>(old);
+        // the atom's frame entry is read before the rank walk and any write (the compiler cannot
+        // tell the arrays apart, and a read after a write waits for it)
         Packed<T> p;
         for (int c = 0; c < 3; ++c)
         {
@@ -1004,6 +1036,12 @@ __device__ __forceinline__ void md_order_body(const StepArgs<T>& a, const Launch
         p.inverse_mass = a.inverse_masses[old];
         p.type = a.types ? a.types[old] : 0;
         p.id = id;
+        const index_t begin = a.bin_offsets[bin];
+        const index_t end = a.bin_offsets[bin + 1];
+        index_t rank = 0;
+        for (index_t k = begin; k < end; ++k) rank += a.ids[a.unordered[k]] < id;
+        const std::int64_t slot = begin + rank;
+        a.order[slot] = static_cast<index_t>(old);
         reinterpret_cast<Packed<T>*>(a.scratch)[slot] = p;
         // the gather copy in the new order, so the fill need not wait for the unpack
         typename Vec4<T>::type p4;
