@@ -204,6 +204,17 @@ __global__ void __launch_bounds__(kBlock) half_list_kernel(
 // registers and writes it once: no atomics. Each pair is met twice, so E and W take half of
 // each meeting. A slot at or past the capacity (an overflowed build) and an id outside [0, N)
 // are skipped rather than read.
+//
+// The walk takes kAhead entries per turn: their ids, then their positions, are all read before
+// any is used (every lane reads, a lane past its row from a harmless address), so a turn waits
+// for memory once rather than once per entry; each entry's terms are then computed on a safe
+// distance and added only for a pair inside the cutoff, in the row's order.
+template <typename T, class Pot>
+constexpr int full_rows_ahead()
+{
+    return sizeof(T) == 4 ? 8 : 4;
+}
+
 template <typename T, class Pot, bool kForces, bool kVirial>
 __global__ void __launch_bounds__(kBlock) full_rows_kernel(
     const T* __restrict__ positions, const T* __restrict__ cell, const index_t* __restrict__ rows,
@@ -211,6 +222,7 @@ __global__ void __launch_bounds__(kBlock) full_rows_kernel(
     const index_t* __restrict__ index, std::int64_t capacity, index_t num_atoms,
     T cutoff_squared, Pot pot, T* __restrict__ forces, double* __restrict__ partials)
 {
+    constexpr int kAhead = full_rows_ahead<T, Pot>();
     __shared__ double shared[16];
     const std::int64_t slices_total = (static_cast<std::int64_t>(num_atoms) + kWave - 1) / kWave;
     // frames of one system (blockIdx.y): each frame's arrays follow the previous frame's (rows
@@ -225,6 +237,8 @@ __global__ void __launch_bounds__(kBlock) full_rows_kernel(
     if (kForces) forces += 3 * frame * num_atoms;
     partials += 8 * frame * gridDim.x;
     const T lx = cell[0], ly = cell[1], lz = cell[2];
+    // a distance inside every potential's domain, for the terms of entries that are not pairs
+    const T safe_r2 = T(0.25) * cutoff_squared;
     const int lane = lane_id();
     T energy = 0;
     T w[6] = {0, 0, 0, 0, 0, 0};
@@ -239,38 +253,62 @@ __global__ void __launch_bounds__(kBlock) full_rows_kernel(
         const index_t count = live ? counts[row] : 0;
         const std::int64_t base = live ? static_cast<std::int64_t>(slices[slice]) + lane : 0;
         const bool own = live && static_cast<std::uint32_t>(i) < static_cast<std::uint32_t>(num_atoms);
-        const T* p = positions + 3LL * (own ? i : 0);
+        const index_t me = own ? i : 0;
+        // the entries this lane walks: its count, cut where its slots reach the capacity
+        std::int64_t walk = own && base < capacity ? count : 0;
+        if (walk > 0)
+        {
+            const std::int64_t fit = (capacity - base + kWave - 1) / kWave;
+            walk = walk < fit ? walk : fit;
+        }
+        const T* p = positions + 3LL * me;
         const T px = p[0], py = p[1], pz = p[2];
         T fx = 0, fy = 0, fz = 0;
-        for (index_t k = 0; k < count; ++k)
+        for (std::int64_t k0 = 0; ballot(k0 < walk) != 0; k0 += kAhead)
         {
-            const std::int64_t slot = base + static_cast<std::int64_t>(k) * kWave;
-            if (!own || slot >= capacity) break;
-            const index_t j = index[slot];
-            if (j == i || static_cast<std::uint32_t>(j) >= static_cast<std::uint32_t>(num_atoms))
-                continue;
-            const T* q = positions + 3LL * j;
-            const T dx = displacement(px, q[0], lx);
-            const T dy = displacement(py, q[1], ly);
-            const T dz = displacement(pz, q[2], lz);
-            const T r2 = exact_squared(dx, dy, dz);
-            if (r2 < cutoff_squared)
+            index_t j[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u)
             {
-                T u, fs;
-                pot.eval(pot.row(i, j), r2, u, fs);
-                energy += T(0.5) * u;
+                const bool take = k0 + u < walk;
+                const index_t v = index[take ? base + (k0 + u) * kWave : 0];
+                j[u] = take ? v : me;
+            }
+            bool valid[kAhead];
+            T qx[kAhead], qy[kAhead], qz[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u)
+            {
+                valid[u] = j[u] != i &&
+                           static_cast<std::uint32_t>(j[u]) < static_cast<std::uint32_t>(num_atoms);
+                const T* q = positions + 3LL * (valid[u] ? j[u] : me);
+                qx[u] = q[0];
+                qy[u] = q[1];
+                qz[u] = q[2];
+            }
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u)
+            {
+                const T dx = displacement(px, qx[u], lx);
+                const T dy = displacement(py, qy[u], ly);
+                const T dz = displacement(pz, qz[u], lz);
+                const T r2 = exact_squared(dx, dy, dz);
+                const bool inside = valid[u] && r2 < cutoff_squared;
+                T e, fs;
+                pot.eval(pot.row(me, valid[u] ? j[u] : me), inside ? r2 : safe_r2, e, fs);
                 const T gx = fs * dx, gy = fs * dy, gz = fs * dz;
-                fx += gx;
-                fy += gy;
-                fz += gz;
+                energy = inside ? energy + T(0.5) * e : energy;
+                fx = inside ? fx + gx : fx;
+                fy = inside ? fy + gy : fy;
+                fz = inside ? fz + gz : fz;
                 if (kVirial)
                 {
-                    w[0] += T(0.5) * dx * gx;
-                    w[1] += T(0.5) * dy * gy;
-                    w[2] += T(0.5) * dz * gz;
-                    w[3] += T(0.5) * dx * gy;
-                    w[4] += T(0.5) * dx * gz;
-                    w[5] += T(0.5) * dy * gz;
+                    w[0] = inside ? w[0] + T(0.5) * dx * gx : w[0];
+                    w[1] = inside ? w[1] + T(0.5) * dy * gy : w[1];
+                    w[2] = inside ? w[2] + T(0.5) * dz * gz : w[2];
+                    w[3] = inside ? w[3] + T(0.5) * dx * gy : w[3];
+                    w[4] = inside ? w[4] + T(0.5) * dx * gz : w[4];
+                    w[5] = inside ? w[5] + T(0.5) * dy * gz : w[5];
                 }
             }
         }
