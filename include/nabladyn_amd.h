@@ -336,7 +336,8 @@ typedef struct ndn_md_list {
                                    to its count rounded up to 32; NULL while stride is 0 */
     int32_t* ids;               /* [N] atom id of each slot */
     int64_t* types;             /* [N] atom type of each slot, or NULL with one type */
-    void* positions4;           /* [N + 1, 4] T: x, y, z, type of each slot (row N spare) */
+    void* positions4;           /* [N + 1, 4] T: x, y, z, type of each slot at the last build,
+                                   the build's own copy (row N spare) */
     void* scratch;              /* [ndn_md_scratch_bytes] */
     uint8_t* scan_temp;
     int64_t scan_temp_bytes;

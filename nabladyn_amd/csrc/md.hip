@@ -149,19 +149,6 @@ struct Vec4<double>
 template <typename T>
 constexpr T kFarAway = T(1e18);
 
-template <typename T>
-__device__ __forceinline__ T reciprocal(T value);
-template <>
-__device__ __forceinline__ float reciprocal<float>(float value)
-{
-    return __builtin_amdgcn_rcpf(value);
-}
-template <>
-__device__ __forceinline__ double reciprocal<double>(double value)
-{
-    return 1.0 / value;
-}
-
 // The force kernel's potentials. A pair's two types arrive as the w of the two gathered
 // float4s (small integers, exact); bind() runs once per kernel and returns what the pair loop
 // evaluates: eval(type_i, type_j, r2, u, fs) with fs = -(du/dr)/r.
@@ -171,6 +158,9 @@ __device__ __forceinline__ double reciprocal<double>(double value)
 template <typename T>
 struct OneTypeLJ
 {
+    // whether eval reads the pair's types (the w of the staged positions)
+    static constexpr bool kTypes = false;
+
     template <typename E>
     static constexpr int waves = sizeof(E) == 4 ? 8 : 1;
 
@@ -206,6 +196,9 @@ struct OneTypeLJ
 template <typename T>
 struct TypedLJ
 {
+    // whether eval reads the pair's types (the w of the staged positions)
+    static constexpr bool kTypes = true;
+
     template <typename E>
     static constexpr int waves = sizeof(E) == 4 ? 5 : 1;
 
@@ -231,6 +224,9 @@ struct TypedLJ
 template <typename T>
 struct TypedTable
 {
+    // whether eval reads the pair's types (the w of the staged positions)
+    static constexpr bool kTypes = true;
+
     template <typename E>
     static constexpr int waves = sizeof(E) == 4 ? 4 : 1;
 
@@ -262,6 +258,9 @@ struct TypedTable
 template <typename T>
 struct OneTypeTable
 {
+    // whether eval reads the pair's types (the w of the staged positions)
+    static constexpr bool kTypes = false;
+
     template <typename E>
     static constexpr int waves = sizeof(E) == 4 ? 8 : 1;
 
@@ -364,6 +363,9 @@ struct SumPart
 template <typename T>
 struct PairSum
 {
+    // whether eval reads the pair's types (the w of the staged positions)
+    static constexpr bool kTypes = true;
+
     template <typename E>
     static constexpr int waves = sizeof(E) == 4 ? 4 : 1;
 
@@ -833,11 +835,6 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
         x[0] = out[0];
         x[1] = out[1];
         x[2] = out[2];
-        // the float4 copy's type (w) is written by the build and stays
-        T* p4 = a.positions4 + 4 * atom;
-        p4[0] = out[0];
-        p4[1] = out[1];
-        p4[2] = out[2];
         if (crossed)
         {
             std::int64_t* image = a.images + 3 * atom;
@@ -2125,20 +2122,35 @@ struct StagedBrick
     }
 };
 
-template <int kThreads, typename T, typename V4>
+// A slot's position as the step's kernels see it, from the frame's positions (current after
+// every drift), with the atom's type as w when the potential reads types (else 0). The float4
+// copy of the frame is written by the build's order kernel for the build's own fill only.
+template <bool kTypes, typename T>
+__device__ __forceinline__ typename Vec4<T>::type current_position(const StepArgs<T>& a,
+                                                                    index_t slot)
+{
+    const T* x = a.positions + 3LL * slot;
+    typename Vec4<T>::type p;
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    p.w = kTypes && a.types != nullptr ? static_cast<T>(a.types[slot]) : T(0);
+    return p;
+}
+
+template <int kThreads, bool kTypes, typename T, typename V4>
 __device__ __forceinline__ StagedBrick<T> stage_brick(const StepArgs<T>& a, index_t brick,
                                                       V4* halo, BrickShared* sh, T reach)
 {
     const index_t dims[3] = {a.nx, a.ny, a.nz};
     const BrickBox box = brick_box(brick, dims, a.brick);
     const BrickFrame<T> frame = brick_frame(a, box, reach);
-    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     // an overflowing halo (the status reports it) loads what its slots hold
     const index_t listed = a.halo_counts[brick];
     const index_t halo_size = listed < a.halo_capacity ? listed : a.halo_capacity;
     const index_t* slots = a.halo_slots + static_cast<std::int64_t>(brick) * a.halo_capacity;
     for (index_t k = threadIdx.x; k < halo_size; k += kThreads)
-        halo[k + 1] = frame.place(pos[slots[k]]);
+        halo[k + 1] = frame.place(current_position<kTypes>(a, slots[k]));
     if (threadIdx.x == 0)
     {
         V4 far;
@@ -2182,7 +2194,6 @@ __device__ __forceinline__ void prune_brick_rows(const StepArgs<T>& a,
                                                  const StagedBrick<T>& staged, const V4* halo)
 {
 #pragma clang fp contract(off)
-    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const T inner_squared = a.inner_radius * a.inner_radius;
     const bool fast = staged.frame.exact;
@@ -2198,7 +2209,7 @@ __device__ __forceinline__ void prune_brick_rows(const StepArgs<T>& a,
     bool live = r < staged.rows;
     index_t slot = live ? staged.slot_of(r) : 0;
     index_t listed = live ? a.counts[slot] : 0;
-    V4 p = pos[slot];
+    V4 p = current_position<false>(a, slot);
     for (index_t first_row = 0; first_row < staged.rows; first_row += kRowsPerPass)
     {
         V4 c = staged.frame.place(p);
@@ -2221,7 +2232,7 @@ __device__ __forceinline__ void prune_brick_rows(const StepArgs<T>& a,
         const bool live_next = r_next < staged.rows;
         const index_t slot_next = live_next ? staged.slot_of(r_next) : 0;
         const index_t listed_next = live_next ? a.counts[slot_next] : 0;
-        const V4 p_next = pos[slot_next];
+        const V4 p_next = current_position<false>(a, slot_next);
         index_t kept = 0;
         // one pass of a row: its 64 entries, kept in (entry of the lane, lane) order
         auto prune_pass = [&](const Pair2 quad, const index_t pass) {
@@ -2294,7 +2305,7 @@ __device__ __forceinline__ void md_prune_body(const StepArgs<T>& a, const Launch
     if (launch.block == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
     // images exact out to the pruned radius
     const StagedBrick<T> staged =
-        stage_brick<kBrickThreads<T>>(a, brick, halo, sh, a.inner_radius);
+        stage_brick<kBrickThreads<T>, false>(a, brick, halo, sh, a.inner_radius);
     prune_brick_rows(a, staged, halo);
 }
 
@@ -2333,11 +2344,13 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
     extern __shared__ __align__(16) unsigned char dynamic_shared[];
     V4* halo = reinterpret_cast<V4*>(dynamic_shared);
     BrickShared* sh = brick_shared<T>(dynamic_shared, a.halo_capacity);
-    double* out = partials + 8LL * launch.block;
+    // the block's sums by column: term k of block b at partials[k * blocks + b]
+    double* out = partials + launch.block;
+    const std::int64_t column = launch.blocks;
     const index_t brick = xcd_brick(static_cast<index_t>(a.bricks), launch);
     if (brick >= a.bricks)
     {
-        if (threadIdx.x < 8) out[threadIdx.x] = 0.0;
+        if (threadIdx.x < 8) out[threadIdx.x * column] = 0.0;
         return;
     }
     // the pruned rows when there are any; a full row that overflowed holds only `stride`
@@ -2354,7 +2367,7 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
     // the cutoff gets the same displacement in either frame); the others out to the cutoff.
     const bool may_refresh = refresh && pruned;
     const bool refreshes = may_refresh && refreshing(a);
-    const StagedBrick<T> staged = stage_brick<kBrickThreads<T>>(
+    const StagedBrick<T> staged = stage_brick<kBrickThreads<T>, Pot::kTypes>(
         a, brick, halo, sh, may_refresh ? static_cast<T>(a.inner_radius) : cutoff);
     if (refreshes && launch.block == 0 && threadIdx.x == 0) ++prune_state(a.header)->refreshes;
     if (refreshes)
@@ -2364,7 +2377,6 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
         // memory; no other block reads these rows)
         __syncthreads();
     }
-    const V4* pos = reinterpret_cast<const V4*>(a.positions4);
     const int sub = lane_id() % G;
     const T lx = a.cell[0], ly = a.cell[1], lz = a.cell[2];
     const bool fast = staged.frame.exact;
@@ -2378,7 +2390,7 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
         const bool live = r < staged.rows;
         const index_t slot = live ? staged.slot_of(r) : 0;
         // a row past the brick's last walks nothing
-        V4 c = staged.frame.place(pos[slot]);
+        V4 c = staged.frame.place(current_position<Pot::kTypes>(a, slot));
         if (!live) c.x = c.y = c.z = -kFarAway<T>;
         const index_t listed_row = live ? counts[slot] : 0;
         const index_t count = listed_row < a.stride ? listed_row : static_cast<index_t>(a.stride);
@@ -2438,8 +2450,8 @@ __device__ __forceinline__ void force_body(const StepArgs<T>& a, const Launch la
     if (threadIdx.x == 0)
     {
         out[0] = e;
-        for (int k = 0; k < 6; ++k) out[1 + k] = v[k];
-        out[7] = k_sum;
+        for (int k = 0; k < 6; ++k) out[(1 + k) * column] = v[k];
+        out[7 * column] = k_sum;
     }
 }
 
@@ -2495,13 +2507,17 @@ __global__ void __launch_bounds__(kBlock) md_entries_kernel(
 constexpr int kFinishThreads = 1024;
 
 // E, W and K from the partials (pairs counted twice, so E and W halve); then the chain's
-// closing half step, whose velocity factor waits in the thermostat for the next step.
+// closing half step, whose velocity factor waits in the thermostat for the next step. The
+// partials are by column (term k of block b at partials[k * blocks + b]), and only the columns
+// the force launch summed are read: E and W when it summed them (`with_terms`), K when the
+// kinetic energy comes from them; the others are zero.
 template <typename T>
 __device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
                                                  const double* __restrict__ partials, int blocks,
-                                                 int kinetic_from_partials)
+                                                 int kinetic_from_partials, int with_terms)
 {
     __shared__ double column[kFinishThreads];
+    __shared__ double totals[8];
     // without a barostat, thread 0 reads the chain and its masses while the partials are summed
     const bool chain_in_registers =
         threadIdx.x == 0 && kinetic_from_partials && a.nvt && !a.barostat;
@@ -2516,43 +2532,51 @@ __device__ __forceinline__ void finish_step_body(const StepArgs<T>& a,
     // writes the potential's coefficients, which a step continuing a run already has. The
     // record of the largest displacement is kept for reading until the next prologue.
     if (threadIdx.x == kWave && kinetic_from_partials && !a.barostat) prepare_checks(a, false);
-    // thread t sums partial (t % 8) of blocks t / 8, t / 8 + 128, ...: coalesced 8-byte reads,
-    // 16 in flight per thread (two rounds of reads at 4,000 blocks), each sum in a fixed order
+    // the columns read, first..last: E (0) and W (1-6) with the terms, K (7) with the kinetic
+    // energy; `span` threads (whole waves) per column, thread r of a column summing its blocks
+    // r, r + span, ... in that order, 16 reads in flight
     constexpr int kInFlight = 16;
-    const int part = threadIdx.x % 8;
-    const std::int64_t values = 8LL * blocks;
+    const int first = with_terms ? 0 : 7;
+    const int last = kinetic_from_partials ? 7 : 6;
+    const int count = last - first + 1;
+    const int span = count > 0 ? kFinishThreads / count / kWave * kWave : kFinishThreads;
+    const int group = threadIdx.x / span;
+    const int r = threadIdx.x % span;
     double sum = 0.0;
-    for (std::int64_t i = threadIdx.x; i < values; i += kInFlight * kFinishThreads)
+    if (group < count)
     {
-        double v[kInFlight];
-#pragma unroll
-        for (int k = 0; k < kInFlight; ++k)
+        const double* values = partials + static_cast<std::int64_t>(first + group) * blocks;
+        for (int b = r; b < blocks; b += kInFlight * span)
         {
-            const std::int64_t at = i + static_cast<std::int64_t>(k) * kFinishThreads;
-            v[k] = at < values ? partials[at] : 0.0;
-        }
+            double v[kInFlight];
 #pragma unroll
-        for (int k = 0; k < kInFlight; ++k)
-            if (i + static_cast<std::int64_t>(k) * kFinishThreads < values) sum += v[k];
+            for (int k = 0; k < kInFlight; ++k)
+            {
+                const int at = b + k * span;
+                v[k] = at < blocks ? values[at] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < kInFlight; ++k)
+                if (b + k * span < blocks) sum += v[k];
+        }
     }
     column[threadIdx.x] = sum;
+    if (threadIdx.x < 8) totals[threadIdx.x] = 0.0;
     __syncthreads();
-    // then partial k over the threads t % 8 == k, in order, by lane k of the first wave (16
-    // columns read before they are added)
-    static_assert(kFinishThreads % (8 * kInFlight) == 0, "whole batches of columns");
-    if (threadIdx.x >= kWave) return;
-    double total = 0.0;
-    if (threadIdx.x < 8)
-        for (int t = part; t < kFinishThreads; t += 8 * kInFlight)
-        {
-            double c[kInFlight];
-#pragma unroll
-            for (int j = 0; j < kInFlight; ++j) c[j] = column[t + 8 * j];
-#pragma unroll
-            for (int j = 0; j < kInFlight; ++j) total += c[j];
-        }
+    // then wave c sums column c's span values: lane l those at l, l + 64, ..., then the lanes
+    // by a fixed butterfly
+    const int wave = threadIdx.x / kWave;
+    if (wave < count)
+    {
+        const int lane = lane_id();
+        double s = 0.0;
+        for (int k = lane; k < span; k += kWave) s += column[wave * span + k];
+        s = wave_sum(s);
+        if (lane == 0) totals[first + wave] = s;
+    }
+    __syncthreads();
     double sums[8];
-    for (int k = 0; k < 8; ++k) sums[k] = read_lane(total, k);
+    for (int k = 0; k < 8; ++k) sums[k] = totals[k];
     if (threadIdx.x != 0) return;
     if (a.energy) a.energy[0] = static_cast<T>(0.5 * sums[0]);
     if (a.virial)
@@ -2612,9 +2636,10 @@ template <typename T>
 __global__ void __launch_bounds__(kFinishThreads) finish_step_kernel(StepArgs<T> a,
                                                                      const double* __restrict__ partials,
                                                                      int blocks,
-                                                                     int kinetic_from_partials)
+                                                                     int kinetic_from_partials,
+                                                                     int with_terms)
 {
-    finish_step_body(a, partials, blocks, kinetic_from_partials);
+    finish_step_body(a, partials, blocks, kinetic_from_partials, with_terms);
 }
 
 // Applies the pending closing factor to the velocities (before they are read out).
@@ -3057,7 +3082,8 @@ int md_step_impl(const ndn_md_system& s, const ndn_md_settings* settings, const 
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, settings->virial != 0,
                                         true, s.partials, s.partial_slots, stream, &blocks, 1);
     if (forces != NDN_OK) return forces;
-    finish_step_kernel<T><<<1, kFinishThreads, 0, stream>>>(a, s.partials, blocks, 1);
+    finish_step_kernel<T><<<1, kFinishThreads, 0, stream>>>(a, s.partials, blocks, 1,
+                                                            settings->virial != 0);
     NDN_LAUNCHED("md_finish");
     return NDN_OK;
 }
@@ -3079,7 +3105,8 @@ int md_evaluate_impl(const ndn_md_system& s, const ndn_potential* pot, const ndn
     const int forces = launch_forces<T>(a, pot, s.coefficients, m.group, virial != 0, kick != 0,
                                         s.partials, s.partial_slots, stream, &blocks);
     if (forces != NDN_OK) return forces;
-    finish_step_kernel<T><<<1, kFinishThreads, 0, stream>>>(a, s.partials, blocks, 0);
+    finish_step_kernel<T><<<1, kFinishThreads, 0, stream>>>(a, s.partials, blocks, 0,
+                                                            virial != 0 || kick == 0);
     NDN_LAUNCHED("md_evaluate_finish");
     return NDN_OK;
 }
@@ -3209,7 +3236,8 @@ struct BatchMember
     T cutoff;
     T cutoff_squared;
     index_t force_blocks;
-    index_t kind;  // BatchPotential
+    index_t kind;        // BatchPotential
+    index_t with_terms;  // its force launch sums E and W (the virial setting)
 };
 
 // A member's bound potential, in the field of its kind.
@@ -3362,7 +3390,7 @@ __global__ void __launch_bounds__(kFinishThreads) batch_finish_kernel(BatchTable
     const BatchBlock b = batch_block(t.map, t.blocks, kPartFinish);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
-    finish_step_body(a, m.partials, m.force_blocks, 1);
+    finish_step_body(a, m.partials, m.force_blocks, 1, m.with_terms);
 }
 
 int batch_potential(const ndn_potential* pot)
@@ -3464,6 +3492,7 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, const int* 
         h.cutoff = static_cast<T>(mb.potential->cutoff);
         h.cutoff_squared = static_cast<T>(mb.potential->cutoff * mb.potential->cutoff);
         h.kind = kinds[i];
+        h.with_terms = mb.settings->virial != 0;
         pots[i] = bind_member<T>(mb.potential, mb.system->coefficients, kinds[i]);
         index_t* own = blocks.data() + static_cast<std::size_t>(i) * kBatchParts;
         member_blocks<T>(h.a, own);

@@ -11,6 +11,21 @@
 
 namespace ndn {
 
+// 1 / value: the hardware reciprocal in fp32 (1 ulp), a division in fp64. For factors that only
+// scale a force.
+template <typename T>
+__device__ __forceinline__ T reciprocal(T value);
+template <>
+__device__ __forceinline__ float reciprocal<float>(float value)
+{
+    return __builtin_amdgcn_rcpf(value);
+}
+template <>
+__device__ __forceinline__ double reciprocal<double>(double value)
+{
+    return 1.0 / value;
+}
+
 // Row of the unordered type pair (a, b): a(2T - a - 1)/2 + b for a <= b
 // (potential/_lennard_jones/_potential.py:143-151).
 __device__ __forceinline__ int pair_row(std::int64_t a, std::int64_t b, int num_types)
