@@ -105,7 +105,8 @@ struct Table
         const bool below = t < T(0);
         u = cubic + (below ? T(0.5) * a * t : T(0));
         const T du_dt = below ? T(0.5) * a : T(0.5) * (a + s * (T(2) * b + T(3) * s * c));
-        fs = -(du_dt / h) / r;
+        // 1/h and 1/r only scale the force: hardware reciprocals
+        fs = -(du_dt * reciprocal(h)) * reciprocal(r);
     }
 };
 
