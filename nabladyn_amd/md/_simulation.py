@@ -753,17 +753,19 @@ class Simulation:
             done += segment
             step = start + done
             if sample_every is not None and step % sample_every == 0:
-                values = self.sample()
-                rows.append([values[name] for name in self.fields])
+                # the row stays on the device; only an echo reads it at once
+                rows.append(self._panel_row())
                 due.append(step)
                 if stream:
-                    stream.write(f"{step:>10d} " + " ".join(f"{v:14.6f}" for v in rows[-1]) + "\n")
+                    stream.write(f"{step:>10d} " + " ".join(f"{v:14.6f}" for v in rows[-1].tolist())
+                                 + "\n")
             if done % _STATUS_INTERVAL == 0 or done == steps:
                 self._check(step)
         self._step += steps
         if sample_every is None:
             return None
-        table = torch.tensor(rows, dtype=torch.float64) if rows else torch.zeros((0, len(self.fields)))
+        table = (torch.stack(rows).cpu() if rows
+                 else torch.zeros((0, len(self.fields)), dtype=torch.float64))
         samples = {"step": torch.tensor(due, dtype=torch.int64)}
         samples.update({name: table[:, k] for k, name in enumerate(self.fields)})
         return samples
@@ -797,39 +799,47 @@ class Simulation:
 
     def sample(self) -> dict[str, float]:
         """The current frame's panel (potential, kinetic, total energy; T; P; NVT: conserved
-        energy). Syncs with the host."""
+        energy; NPT: volume). Syncs with the host."""
+        return dict(zip(self.fields, self._panel_row().tolist()))
+
+    def _panel_row(self) -> Tensor:
+        """The current frame's panel as a float64 device tensor in ``fields`` order, queued on
+        the current stream with no host sync (``md/_simulation.py:366-402`` keeps the panel on
+        the device the same way)."""
         self._require()
         self._settle()
         self._evaluate(virial=True)
         masses = self.slot_masses.double()
-        kinetic = float(0.5 * (masses[:, None] * self.velocities.double().square()).sum())
-        energy = float(self.energy)
-        volume = float(self.cell.double().prod())
-        values = {
-            "potential_energy": energy,
-            "kinetic_energy": kinetic,
-            "total_energy": energy + kinetic,
-            "temperature": 2.0 * kinetic / (self._degrees_of_freedom * BOLTZMANN),
-            "pressure": (2.0 * kinetic + float(self.virial.double().trace())) / (3.0 * volume),
-        }
+        kinetic = 0.5 * (masses[:, None] * self.velocities.double().square()).sum()
+        energy = self.energy.double()
+        volume = self.cell.double().prod()
+        values = [
+            energy,
+            kinetic,
+            energy + kinetic,
+            2.0 * kinetic / (self._degrees_of_freedom * BOLTZMANN),
+            (2.0 * kinetic + self.virial.double().trace()) / (3.0 * volume),
+        ]
         if self._nvt:
             q, p = self.thermostat[0:3].double(), self.thermostat[3:6].double()
-            bath = float((p.square() / (2.0 * self.chain_masses)).sum()
-                         + self._degrees_of_freedom * self._kT * q[0] + self._kT * q[1:].sum())
-            values["conserved_energy"] = energy + kinetic + bath
+            bath = ((p.square() / (2.0 * self.chain_masses)).sum()
+                    + self._degrees_of_freedom * self._kT * q[0] + self._kT * q[1:].sum())
+            conserved = energy + kinetic + bath
         if self._npt:
             # md/_ensembles/_npt.py:205-230: plus the barostat's chain (one degree of freedom),
             # P V of the master cell and the box's kinetic energy
             b = self.barostat.double()
             box_q, box_p = b[_BOX_Q:_BOX_Q + 3], b[_BOX_PC:_BOX_PC + 3]
             box_masses = b[_BOX_MASS:_BOX_MASS + 3]
-            box_bath = float((box_p.square() / (2.0 * box_masses)).sum()
-                             + self._kT * box_q[0] + self._kT * box_q[1:].sum())
-            master_volume = float(b[_MASTER_CELL:_MASTER_CELL + 3].prod())
-            values["volume"] = master_volume
-            values["conserved_energy"] += (box_bath + self._pressure * master_volume
-                                           + float(b[_BOX_P]) ** 2 / (2.0 * self._box_mass))
-        return values
+            box_bath = ((box_p.square() / (2.0 * box_masses)).sum()
+                        + self._kT * box_q[0] + self._kT * box_q[1:].sum())
+            master_volume = b[_MASTER_CELL:_MASTER_CELL + 3].prod()
+            values.append(master_volume)
+            conserved = conserved + (box_bath + self._pressure * master_volume
+                                     + b[_BOX_P].square() / (2.0 * self._box_mass))
+        if self._nvt:
+            values.append(conserved)
+        return torch.stack(values)
 
     def state(self) -> State:
         """A copy of the current frame (velocities with the pending chain factor applied)."""
