@@ -103,7 +103,9 @@ __global__ void __launch_bounds__(kBlock) batch_assign_kernel(BatchArgs<T> a)
     const index_t nx = a.dims[3 * s], ny = a.dims[3 * s + 1];
     const index_t bin = a.bin_start[s] + b[0] + nx * (b[1] + ny * b[2]);
     a.bin_of[atom] = bin;
-    atomicAdd(a.bin_counts + bin, 1);
+    // one atomic per run of a bin's atoms in neighbouring lanes
+    const LaneRun run = equal_runs(bin);
+    if (run.head) atomicAdd(a.bin_counts + bin, run.length);
 }
 
 template <typename T>
@@ -112,7 +114,13 @@ __global__ void __launch_bounds__(kBlock) batch_scatter_kernel(BatchArgs<T> a)
     const std::int64_t atom = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (atom >= a.num_atoms) return;
     const index_t bin = a.bin_of[atom];
-    a.unordered[a.bin_offsets[bin] + atomicSub(a.bin_counts + bin, 1) - 1] =
+    // one atomic per run: the run's lanes take the places below the count it returns, one each
+    // (batch_order_kernel orders a bin's atoms by index after)
+    const LaneRun run = equal_runs(bin);
+    index_t count = 0;
+    if (run.head) count = atomicSub(a.bin_counts + bin, run.length);
+    count = __shfl(count, run.first, kWave);
+    a.unordered[a.bin_offsets[bin] + count - 1 - (lane_id() - run.first)] =
         static_cast<index_t>(atom);
 }
 

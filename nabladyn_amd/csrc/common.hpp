@@ -93,6 +93,34 @@ __device__ __forceinline__ V wave_sum(V value)
     return value;
 }
 
+// Runs of equal keys in neighbouring lanes (the active lanes a prefix of the wave, as in a
+// grid-stride loop over consecutive items): whether this lane starts its run, the run's length
+// (for a starting lane) and the run's first lane (for every lane). Binning counts with one
+// atomic per run of atoms in one bin: a frame sorted at the last build, or atoms given in a
+// spatial order, come in such runs.
+struct LaneRun
+{
+    bool head;
+    int length;
+    int first;
+};
+
+__device__ __forceinline__ LaneRun equal_runs(index_t key)
+{
+    const int lane = lane_id();
+    const std::uint64_t active = ballot(true);
+    const index_t before = __shfl_up(key, 1, kWave);
+    LaneRun run;
+    run.head = lane == 0 || before != key;
+    // inactive lanes end runs too
+    const std::uint64_t heads = ballot(run.head) | ~active;
+    const std::uint64_t higher = lane == kWave - 1 ? 0ull : heads >> (lane + 1);
+    run.length = (higher == 0 ? kWave : lane + 1 + __builtin_ctzll(higher)) - lane;
+    const std::uint64_t lower = lane == kWave - 1 ? heads : heads & ((2ull << lane) - 1ull);
+    run.first = 63 - __builtin_clzll(lower);
+    return run;
+}
+
 template <typename V>
 __device__ __forceinline__ V wave_max(V value)
 {

@@ -919,34 +919,6 @@ __device__ __forceinline__ std::int64_t row_first_bin(index_t y, index_t z, inde
             static_cast<std::int64_t>(y) * depth + layer);
 }
 
-// Runs of equal keys in neighbouring lanes (the active lanes a prefix of the wave, as in a
-// grid-stride loop over consecutive slots): whether this lane starts its run, the run's length
-// (for a starting lane) and the run's first lane (for every lane). The slots of a frame sorted
-// at the last build stay mostly in their sub-bins, so a wave's atoms come in runs of one
-// sub-bin, and one atomic per run stands for the run's atoms.
-struct LaneRun
-{
-    bool head;
-    int length;
-    int first;
-};
-
-__device__ __forceinline__ LaneRun equal_runs(index_t key)
-{
-    const int lane = lane_id();
-    const std::uint64_t active = ballot(true);
-    const index_t before = __shfl_up(key, 1, kWave);
-    LaneRun run;
-    run.head = lane == 0 || before != key;
-    // inactive lanes end runs too
-    const std::uint64_t heads = ballot(run.head) | ~active;
-    const std::uint64_t higher = lane == kWave - 1 ? 0ull : heads >> (lane + 1);
-    run.length = (higher == 0 ? kWave : lane + 1 + __builtin_ctzll(higher)) - lane;
-    const std::uint64_t lower = lane == kWave - 1 ? heads : heads & ((2ull << lane) - 1ull);
-    run.first = 63 - __builtin_clzll(lower);
-    return run;
-}
-
 template <typename T>
 __device__ __forceinline__ void md_assign_body(const StepArgs<T>& a, const Launch launch,
                                                int forced)

@@ -128,7 +128,9 @@ __global__ void __launch_bounds__(kBlock) assign_bins_kernel(
     const index_t bin = axis_bin(p[0], widths[0], nx) +
                         nx * (axis_bin(p[1], widths[1], ny) + ny * axis_bin(p[2], widths[2], dims[2]));
     atom_bin[atom] = bin;
-    atomicAdd(bin_counts + bin, 1);
+    // one atomic per run of a bin's atoms in neighbouring lanes
+    const LaneRun run = equal_runs(bin);
+    if (run.head) atomicAdd(bin_counts + bin, run.length);
 }
 
 // Places each atom in its bin's slice in the order of the atomics; the counts return to zero.
@@ -140,7 +142,13 @@ __global__ void __launch_bounds__(kBlock) scatter_atoms_kernel(
     const std::int64_t atom = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (atom >= num_atoms || !rebuilding(rebuild, status)) return;
     const index_t bin = atom_bin[atom];
-    unordered_ids[bin_offsets[bin] + atomicSub(bin_counts + bin, 1) - 1] =
+    // one atomic per run of a bin's atoms: the run's lanes take the places below the count it
+    // returns, one each (order_bins_kernel ranks a bin's atoms by id after)
+    const LaneRun run = equal_runs(bin);
+    index_t count = 0;
+    if (run.head) count = atomicSub(bin_counts + bin, run.length);
+    count = __shfl(count, run.first, kWave);
+    unordered_ids[bin_offsets[bin] + count - 1 - (lane_id() - run.first)] =
         static_cast<index_t>(atom);
 }
 
