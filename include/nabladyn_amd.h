@@ -446,7 +446,9 @@ typedef struct ndn_md_batch {
     int32_t kinds;        /* bit k: some member binds potential kind k (a force launch) */
     int32_t group;        /* every member's lanes per row */
     int32_t virial;       /* every member's ndn_md_settings.virial */
-    int32_t reserved;
+    int32_t continues;    /* set by the caller: 1 when every member's previous step was a
+                             batch step of this batch (its finish opened this step), so the
+                             prologue launch is skipped; 0 (prepare's value) runs it */
     int64_t members_offset;
     int64_t potentials_offset;
     int64_t blocks_offset;

@@ -3352,8 +3352,9 @@ __global__ void __launch_bounds__(kBrickThreads<T>, (kForceWaves<T, Pot, kVirial
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
     const Pot pot = pick(pots[b.member], static_cast<const Pot*>(nullptr));
+    // each member's force blocks refresh its bricks' pruned rows when the member refreshes
     force_body<T, Pot, G, kVirial, true>(a, b.launch, pot, m.cutoff, m.cutoff_squared,
-                                         m.half_timestep, m.partials);
+                                         m.half_timestep, m.partials, 1);
 }
 
 template <typename T>
@@ -3591,9 +3592,13 @@ template <typename T>
 int batch_step_impl(const ndn_md_batch& b, hipStream_t stream)
 {
     auto grid = [&](int launch) { return static_cast<unsigned>(b.grids[launch]); };
-    batch_prologue_kernel<T><<<grid(kLaunchPrologue), kBlock, 0, stream>>>(
-        batch_table<T>(b, kLaunchPrologue));
-    NDN_LAUNCHED("md_batch_prologue");
+    // a step continuing a run was opened by the previous step's finish, as in md_step_impl
+    if (!b.continues)
+    {
+        batch_prologue_kernel<T><<<grid(kLaunchPrologue), kBlock, 0, stream>>>(
+            batch_table<T>(b, kLaunchPrologue));
+        NDN_LAUNCHED("md_batch_prologue");
+    }
     batch_first_half_kernel<T><<<grid(kLaunchFirstHalf), kBlock, 0, stream>>>(
         batch_table<T>(b, kLaunchFirstHalf));
     NDN_LAUNCHED("md_batch_first_half");
@@ -3612,14 +3617,7 @@ int batch_step_impl(const ndn_md_batch& b, hipStream_t stream)
     batch_unpack_fill_kernel<T><<<grid(kLaunchUnpackFill), kBlock, 0, stream>>>(
         batch_table<T>(b, kLaunchUnpackFill));
     NDN_LAUNCHED("md_batch_unpack_fill");
-    if (b.grids[kLaunchPrune] > 0)
-    {
-        const std::size_t lds = static_cast<std::size_t>(b.lds[kLaunchPrune]);
-        allow_shared(batch_prune_kernel<T>, lds);
-        batch_prune_kernel<T><<<grid(kLaunchPrune), kBrickThreads<T>, lds, stream>>>(
-            batch_table<T>(b, kLaunchPrune));
-        NDN_LAUNCHED("md_batch_prune");
-    }
+    // the pruned rows are refreshed by the force launches (force_body's refresh)
     for (int kind = 0; kind < kBatchPotentials; ++kind)
     {
         if (b.grids[kLaunchForce + kind] == 0) continue;
@@ -3908,7 +3906,7 @@ int ndn_md_batch_prepare(ndn_md_batch* batch, const ndn_md_member* members, int3
     batch->count = count;
     batch->group = members[0].md_list->group;
     batch->virial = members[0].settings->virial != 0;
-    batch->reserved = 0;
+    batch->continues = 0;
     const std::int64_t staged_bytes = static_cast<std::int64_t>(staged);
     if (dtype == NDN_F32)
         return prepare_batch<float>(batch, members, kinds.data(), count, staged_bytes, stream);

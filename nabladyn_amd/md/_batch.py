@@ -45,7 +45,7 @@ class _Batch(_native.ctypes.Structure):
         ("table", _native.ctypes.c_void_p), ("table_bytes", _native.ctypes.c_int64),
         ("dtype", _native.ctypes.c_int32), ("count", _native.ctypes.c_int32),
         ("kinds", _native.ctypes.c_int32), ("group", _native.ctypes.c_int32),
-        ("virial", _native.ctypes.c_int32), ("reserved", _native.ctypes.c_int32),
+        ("virial", _native.ctypes.c_int32), ("continues", _native.ctypes.c_int32),
         ("members_offset", _native.ctypes.c_int64),
         ("potentials_offset", _native.ctypes.c_int64), ("blocks_offset", _native.ctypes.c_int64),
         ("offsets", _native.ctypes.c_int64 * _KERNELS), ("grids", _native.ctypes.c_int64 * _KERNELS),
@@ -124,8 +124,12 @@ class SimulationBatch:
         stream = _native.stream(self._device)
         handle = _native.ctypes.byref(self._batch)
         step = _native.lib().ndn_md_batch_step
-        for _ in range(steps):
+        # the first step of a call runs the prologue; each later one was opened by the step
+        # before it (its finish)
+        for k in range(steps):
+            self._batch.continues = 1 if k > 0 else 0
             _native.check("md_batch_step", step(handle, stream))
+        self._batch.continues = 0
         for simulation in self.simulations:
             simulation._step += steps
 
