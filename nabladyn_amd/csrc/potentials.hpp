@@ -60,7 +60,8 @@ struct LJ
     __device__ __forceinline__ void eval(int row_index, T r2, T& u, T& fs) const
     {
         const LJRow<T> c = rows[row_index];
-        const T inverse = T(1) / r2;
+        // 1/r^2 by the hardware reciprocal (1 ulp in fp32), as the MD kernels take it
+        const T inverse = reciprocal(r2);
         const T s = c.sigma_squared * inverse;
         const T p6 = s * s * s;
         u = c.four_eps * (p6 * p6 - p6) - c.shift;
