@@ -92,6 +92,23 @@ def test_batch_then_alone_continues(device):
         np.testing.assert_equal(a.sample()["temperature"], b.sample()["temperature"])
 
 
+def test_batch_single_steps_equal_a_run(device):
+    """A call of one step runs the prologue; later steps of a call are opened by the step
+    before them. Either way each member ends bitwise where it ends alone."""
+    makers = _members("table", torch.float32, device)[:3]
+    alone = [make() for make in makers]
+    for s in alone:
+        s.run(45)
+    batched = [make(graphs=False) for make in makers]
+    batch = md.SimulationBatch(batched)
+    for _ in range(15):
+        batch.advance(1)
+    batch.advance(30)
+    batch.check()
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        assert _same(a, b), k
+
+
 def test_mixed_potentials_in_one_batch(device):
     # LJ and one-type-table members share every launch but their force launches
     makers = _members("lj", torch.float32, device)[:3] + _members("table", torch.float32, device)[2:]
