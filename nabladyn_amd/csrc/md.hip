@@ -457,6 +457,7 @@ struct StepArgs
     T* reuse;
     index_t* rebuild;
     index_t* rebuild_count;
+    index_t* batch_rebuild;  // a batch member: the batch's flag of any member rebuilding
     // list
     index_t* status;
     index_t* dims_scratch;
@@ -866,7 +867,11 @@ __device__ __forceinline__ void first_half_body(const StepArgs<T>& a, const Laun
     if (lane_id() == 0)
     {
         block_max[wave] = wave_moved;
-        if (check && wave_moved >= reuse[0] && a.rebuild[0] == 0) a.rebuild[0] = 1;
+        if (check && wave_moved >= reuse[0] && a.rebuild[0] == 0)
+        {
+            a.rebuild[0] = 1;
+            if (a.batch_rebuild) a.batch_rebuild[0] = 1;
+        }
         if (check && a.inner_index)
         {
             PruneState* prune = prune_state(a.header);
@@ -3270,7 +3275,17 @@ struct BatchTable
     const BatchMember<T>* __restrict__ members;
     const index_t* __restrict__ blocks;  // [member][part]
     const std::uint32_t* __restrict__ map;
+    // set when some member rebuilds this step (by the first half), cleared by the finish: the
+    // gated build launches' blocks read it before their member's arguments
+    index_t* any_rebuild;
 };
+
+// Whether the gated build launches have work this step: one read of the batch's flag.
+template <typename T>
+__device__ __forceinline__ bool batch_builds(const BatchTable<T>& t)
+{
+    return t.any_rebuild[0] != 0;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_prologue_kernel(BatchTable<T> t)
@@ -3288,6 +3303,9 @@ __global__ void __launch_bounds__(kBlock) batch_first_half_kernel(BatchTable<T> 
     const BatchBlock b = batch_block(t.map, t.blocks, kPartFirstHalf);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
+    // a rebuild decided before this launch (the prologue's checks) raises the batch's flag
+    // here; one the first half decides raises it where it is decided
+    if (b.launch.block == 0 && threadIdx.x == 0 && a.rebuild[0] != 0) t.any_rebuild[0] = 1;
     first_half_body(a, b.launch, m.timestep, m.half_timestep);
 }
 
@@ -3296,6 +3314,7 @@ __global__ void __launch_bounds__(kBlock) batch_first_half_kernel(BatchTable<T> 
     template <typename T>                                                                        \
     __global__ void __launch_bounds__(kBlock) batch_##name##_kernel(BatchTable<T> t)             \
     {                                                                                            \
+        if (!batch_builds(t)) return;                                                            \
         const BatchBlock b = batch_block(t.map, t.blocks, part);                                 \
         const StepArgs<T> a = t.members[b.member].a;                                             \
         name##_body(a, b.launch, 0);                                                             \
@@ -3307,6 +3326,7 @@ NDN_BATCH_GATED(md_order, kPartOrder)
 template <typename T>
 __global__ void __launch_bounds__(kListScanThreads) batch_scan_kernel(BatchTable<T> t)
 {
+    if (!batch_builds(t)) return;
     const BatchBlock b = batch_block(t.map, t.blocks, kPartScan);
     const StepArgs<T> a = t.members[b.member].a;
     if (!rebuilding(a.rebuild, a.status)) return;
@@ -3316,6 +3336,7 @@ __global__ void __launch_bounds__(kListScanThreads) batch_scan_kernel(BatchTable
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_scatter_halo_kernel(BatchTable<T> t)
 {
+    if (!batch_builds(t)) return;
     const BatchBlock b = batch_block(t.map, t.blocks, kPartScatter);
     const StepArgs<T> a = t.members[b.member].a;
     const index_t scatter = b.launch.blocks;
@@ -3326,6 +3347,7 @@ __global__ void __launch_bounds__(kBlock) batch_scatter_halo_kernel(BatchTable<T
 template <typename T>
 __global__ void __launch_bounds__(kBlock) batch_unpack_fill_kernel(BatchTable<T> t)
 {
+    if (!batch_builds(t)) return;
     const BatchBlock b = batch_block(t.map, t.blocks, kPartUnpack);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
@@ -3363,6 +3385,8 @@ __global__ void __launch_bounds__(kFinishThreads) batch_finish_kernel(BatchTable
     const BatchBlock b = batch_block(t.map, t.blocks, kPartFinish);
     const BatchMember<T>& m = t.members[b.member];
     const StepArgs<T> a = m.a;
+    // every gated build launch of this step has run: the flag starts the next step clear
+    if (b.member == 0 && threadIdx.x == 0) t.any_rebuild[0] = 0;
     finish_step_body(a, m.partials, m.force_blocks, 1, m.with_terms);
 }
 
@@ -3489,6 +3513,8 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, const int* 
         batch->offsets[k] = offset;
         offset += align_table(4 * std::max<std::int64_t>(grids[k], 1));
     }
+    const std::int64_t flag_at = offset;  // batch_flag_offset's answer
+    offset += align_table(sizeof(index_t));
     const std::int64_t needed = offset;
     batch->members_offset = 0;
     batch->potentials_offset = pots_at;
@@ -3511,6 +3537,9 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, const int* 
         return NDN_OK;
     }
     NDN_REQUIRE(batch->table_bytes >= needed, "the batch table is too small");
+    for (int i = 0; i < count; ++i)
+        hosts[i].a.batch_rebuild =
+            reinterpret_cast<index_t*>(static_cast<std::uint8_t*>(batch->table) + flag_at);
     std::vector<std::uint8_t> image(static_cast<std::size_t>(needed), 0);
     std::memcpy(image.data(), hosts.data(), sizeof(BatchMember<T>) * count);
     std::memcpy(image.data() + pots_at, pots.data(), sizeof(MemberPotential<T>) * count);
@@ -3545,13 +3574,22 @@ int prepare_batch(ndn_md_batch* batch, const ndn_md_member* members, const int* 
     return NDN_OK;
 }
 
+// The batch's rebuild flag: the table's slot after the last launch's block map (prepare_batch's
+// layout).
+std::int64_t batch_flag_offset(const ndn_md_batch& b)
+{
+    const int last = kBatchLaunches - 1;
+    return b.offsets[last] + align_table(4 * std::max<std::int64_t>(b.grids[last], 1));
+}
+
 template <typename T>
 BatchTable<T> batch_table(const ndn_md_batch& b, int launch)
 {
     auto* base = static_cast<std::uint8_t*>(b.table);
     return BatchTable<T>{reinterpret_cast<const BatchMember<T>*>(base + b.members_offset),
                          reinterpret_cast<const index_t*>(base + b.blocks_offset),
-                         reinterpret_cast<const std::uint32_t*>(base + b.offsets[launch])};
+                         reinterpret_cast<const std::uint32_t*>(base + b.offsets[launch]),
+                         reinterpret_cast<index_t*>(base + batch_flag_offset(b))};
 }
 
 template <typename T, class Pot, int G, bool kVirial>
