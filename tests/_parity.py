@@ -1,6 +1,6 @@
 """Shared parity helpers for the table tests (imported by tests, collected by none).
 
-The north-star bar for fp32 forces and virials: within 1e-5 relative of the reference, with
+The bar for fp32 forces and virials: within 1e-5 relative of the reference, with
 each result's error against an fp64 evaluation stated beside it. "The reference" for a table is
 its own fp32 route, restated by the oracle: ``Tabulated._energy_wrapped``
 (``potential/_tabulated/_potential.py:146-162``) differentiated by ``torch.func`` as
